@@ -1,0 +1,195 @@
+/*
+ * bbdg.h -- C ABI of libbbdg.so: Bernstein-Bezier DG operators on MI355X (gfx950).
+ *
+ * The entry points replace the Julia method table that BernsteinBasis.jl exports
+ * (src/BernsteinBasis.jl:19-23) and the right-hand side that the examples assemble
+ * from it (examples/3d_wave.jl:36-84, examples/3d_maxwell.jl:180-237).  Each
+ * declaration names the reference interface it replaces.
+ *
+ * Conventions
+ *  - All operator pointers are DEVICE pointers unless the name says _host.
+ *  - Indices are 0-based.  The reference's 1-based tables are exported as they are.
+ *  - A "column" is one element x one field.  A bb_layout gives the strides (in
+ *    array elements, not bytes) of the element, node and field indices, so both the
+ *    reference's SVector{F} column-major layout ([K][Np][F]: field 1, node F,
+ *    element Np*F) and the solver's SoA layout ([K][F][Np]: node 1, field Np,
+ *    element F*Np) are accepted without copies.
+ *  - Every function returns BB_OK (0) or an error code; nothing aborts or throws
+ *    across the ABI.  bb_last_error() gives the message of the last failure on a
+ *    context.  The reference performs no checks on the hot path (@inbounds).
+ *  - A context owns one HIP stream; all work is ordered on it.  Contexts are not
+ *    re-entrant: use one per host thread.
+ */
+#ifndef BBDG_H
+#define BBDG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum bb_status {
+    BB_OK = 0,
+    BB_ERR_ARG = 1,         /* bad enum / degree / count / stride */
+    BB_ERR_NULL = 2,        /* null pointer */
+    BB_ERR_HIP = 3,         /* HIP runtime error (message in bb_last_error) */
+    BB_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
+    BB_ERR_STATE = 5        /* call out of order (e.g. graph launch before capture) */
+};
+
+enum bb_dtype { BB_F64 = 0, BB_F32 = 1 };
+enum bb_axis { BB_AXIS_R = 0, BB_AXIS_S = 1, BB_AXIS_T = 2 };
+enum bb_pde { BB_PDE_WAVE = 0, BB_PDE_MAXWELL = 1 };
+/* interface flux: the reference's formula verbatim, or the same upwind flux with all
+ * terms scaled by the face Jacobian Jf = |nJ| (stable under explicit RK on fine meshes) */
+enum bb_flux { BB_FLUX_REFERENCE = 0, BB_FLUX_UPWIND = 1 };
+/* lift arithmetic for FP32 data: everything FP32; FP32 chain + FP64 weighted
+ * accumulation; FP32 chain carried as float-float (compensated, two-sum) */
+enum bb_lift_mode { BB_LIFT_PLAIN = 0, BB_LIFT_ACC64 = 1, BB_LIFT_TWOSUM = 2 };
+/* solver arithmetic: FP64 everywhere; mixed = FP32 derivative/flux/L0/chain, FP64 lift
+ * accumulation and FP64 solution accumulator (FP32 RK residual register) */
+enum bb_prec { BB_PREC_F64 = 0, BB_PREC_MIXED = 1 };
+/* solver right-hand-side path: one kernel per operator (the reference's structure,
+ * FP64 only) or one fused RHS + LSRK stage kernel */
+enum bb_path { BB_PATH_PLAIN = 0, BB_PATH_FUSED = 1 };
+
+typedef struct bb_layout {
+    int64_t elem;  /* stride between elements */
+    int64_t node;  /* stride between nodes (volume) or face nodes (face data) */
+    int64_t field; /* stride between fields */
+} bb_layout;
+
+typedef struct bb_ctx bb_ctx;
+typedef struct bb_solver bb_solver;
+
+/* ------------------------------------------------------------------ context */
+int bb_ctx_create(int device, bb_ctx **out);
+int bb_ctx_destroy(bb_ctx *ctx);
+const char *bb_last_error(const bb_ctx *ctx);
+const char *bb_version(void);
+int bb_device_count(int *count);
+/* gcnArchName, name and CU count of the context's device */
+int bb_device_info(bb_ctx *ctx, char *arch, int arch_len, char *name, int name_len, int *cu_count);
+/* HIP stream (hipStream_t) of the context, for callers that order their own work on it */
+int bb_ctx_stream(bb_ctx *ctx, void **stream);
+
+/* ------------------------------------------------------------------ memory (device) */
+int bb_malloc(bb_ctx *ctx, size_t bytes, void **dptr);
+int bb_free(bb_ctx *ctx, void *dptr);
+int bb_memcpy_h2d(bb_ctx *ctx, void *dst, const void *src_host, size_t bytes);
+int bb_memcpy_d2h(bb_ctx *ctx, void *dst_host, const void *src, size_t bytes);
+int bb_memcpy_d2d(bb_ctx *ctx, void *dst, const void *src, size_t bytes);
+int bb_fill_zero(bb_ctx *ctx, void *dst, size_t bytes); /* kernel fill, capturable */
+int bb_synchronize(bb_ctx *ctx);
+
+/* HIP-event timing on the context stream.  bb_timer_stop returns the milliseconds
+ * between the matching start and stop. */
+int bb_timer_start(bb_ctx *ctx);
+int bb_timer_stop(bb_ctx *ctx, float *ms);
+
+/* ------------------------------------------------------------------ host tables
+ * (no device work; bit-exact with the reference) */
+
+/* src/ordering.jl:8-16 tri_offsets(N) and :18-26 tet_offsets(N) (21 entries each) and
+ * src/constants.jl:10-12 l_j(N) (20 entries).  1 <= N <= 20. */
+int bb_tables(int N, int64_t tri_offsets[21], int64_t tet_offsets[21], double l_j[20]);
+/* src/ordering.jl:37-39 linear_to_ijkl_lookup(N): out[4*n+0..3] = (i,j,k,l), 0 <= N <= 20 */
+int bb_linear_to_ijkl(int N, int32_t *out_host);
+/* src/ordering.jl:47-51 bernstein_2d_scalar_multiindex_lookup(N)[1]: out[3*m..] = (i,j,k) */
+int bb_linear_to_ijk_2d(int N, int32_t *out_host);
+/* face traces (the role of rd.Fmask at examples/3d_wave.jl:41): out[f*Nfp + m] is the
+ * volume index of face f's node m, faces f1..f4 in the order the lift expects
+ * (src/BernsteinLift.jl:207-217) */
+int bb_fmask(int N, int32_t *out_host);
+/* dense entries: src/BernsteinDerivativeMatrix_3D.jl:69-94 getindex(D{r,s,t}, m, n) and
+ * src/ElevationMatrix.jl:17-24 getindex(ElevationMatrix{N}, m, n) */
+int bb_deriv_entry(int axis, int N, int m, int n, double *value);
+int bb_elevation_entry(int N, int m, int n, double *value);
+
+/* ------------------------------------------------------------------ operators
+ * All act on K elements x F fields ("columns"), device pointers, ordered on ctx. */
+
+/* mul!(out, D{r,s,t}, x), src/BernsteinDerivativeMatrix_3D.jl:134-136, :176-178,
+ * :218-220 (vector) and :222-231 (column-batched); x, out: Np values per column. */
+int bb_deriv_apply(bb_ctx *ctx, int axis, int N, int F, int64_t K, int dtype, const void *x, bb_layout lx,
+                   void *out, bb_layout lo);
+
+/* The three derivative matvecs combined with per-element geometric factors
+ * (examples/3d_wave.jl:63-71 + :75-78): dx = rxJ Dr x + sxJ Ds x + txJ Dt x, etc.
+ * geo: [K][9] FP64 = rxJ sxJ txJ ryJ syJ tyJ rzJ szJ tzJ. */
+int bb_grad_apply(bb_ctx *ctx, int N, int F, int64_t K, int dtype, const void *x, bb_layout lx,
+                  const double *geo, void *dx, void *dy, void *dz, bb_layout lo);
+
+/* mul!(out, L::BernsteinLift, x), src/BernsteinLift.jl:191-220 (vector), :264-269
+ * (matrix), threaded_mul! :233-262: x holds 4*Nfp face values per column in the order
+ * [f1; f2; f3; f4] (node stride lx.node), out Np values.  mode applies to BB_F32. */
+int bb_lift_apply(bb_ctx *ctx, int N, int F, int64_t K, int dtype, int mode, const void *x, bb_layout lx,
+                  void *out, bb_layout lo);
+
+/* face_lift_multiply!(out, L, x), src/BernsteinLift.jl:139-179: one face, Nfp -> Np
+ * (slab k = j holds l_j (E^N_{N-j})^T L0 x).  x_dtype / out_dtype may differ (FP32 in,
+ * FP64 out measures the chain precision); mode applies to FP32 input. */
+int bb_face_lift_apply(bb_ctx *ctx, int N, int F, int64_t K, int x_dtype, int out_dtype, int mode,
+                       const void *x, bb_layout lx, void *out, bb_layout lo);
+
+/* ElevationMatrix{N} * x, src/ElevationMatrix.jl:11-24 (degree N-1 -> N on a face;
+ * the same product as elevation_multiply!(out, N-1, x), src/BernsteinLift.jl:100-125).
+ * x: Nfp(N-1) values per column, out: Nfp(N).  1 <= N <= 20. */
+int bb_elevate_apply(bb_ctx *ctx, int N, int F, int64_t K, int dtype, const void *x, bb_layout lx, void *out,
+                     bb_layout lo);
+
+/* uM .= view(q, rd.Fmask, :), examples/3d_wave.jl:41 (out: 4*Nfp face values per column) */
+int bb_face_gather(bb_ctx *ctx, int N, int F, int64_t K, int dtype, const void *q, bb_layout lq, void *uM,
+                   bb_layout lm);
+
+/* interface flux loop, examples/3d_wave.jl:43-56 (wave) / 3d_maxwell.jl:187-209 (Maxwell).
+ * uM, flux: 4*Nfp face values per element x F fields; mapP: [K][4Nfp] indices into the
+ * flattened face-node array (StartUpDG md.mapP, 0-based); bc: [K][4Nfp] 0 = interior,
+ * 1 = wall (uP = bc_sign .* uM, may be NULL); nxJ/nyJ/nzJ: [K][4Nfp] FP64. FP64 only. */
+int bb_interface_flux(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *uM, bb_layout lm,
+                      const int32_t *mapP, const int8_t *bc, const double *bc_sign_host, const double *nxJ,
+                      const double *nyJ, const double *nzJ, double *flux, bb_layout lf);
+
+/* ------------------------------------------------------------------ solver
+ * Device-resident mesh + state for the wave (F=4: p,u,v,w) or Maxwell (F=6: Hx,Hy,Hz,
+ * Ex,Ey,Ez) system on K affine tets of degree N (1 <= N <= 10).  Replaces the
+ * rhs_matvec! + ODEProblem pair of examples/3d_wave.jl:36-134 / 3d_maxwell.jl:180-282
+ * with a five-stage low-storage RK45 (Carpenter & Kennedy 1994).
+ *
+ * Host arrays:  geo [K][9], J [K], nJ [K][4][3] (nxJ nyJ nzJ per face, faces f1..f4),
+ *               nbr [K][4] neighbour element, nbr_face [K][4] (0..3),
+ *               perm [K][4] (0..5: neighbour face vertex m = my face vertex p[m], p in
+ *               lexicographic order of the permutations of {0,1,2}), bc [K][4]
+ *               (0 interior, 1 wall), bc_sign [F] (uP = bc_sign .* uM on walls).
+ * State layout: [K][F][Np] FP64. */
+int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *geo_host,
+                     const double *J_host, const double *nJ_host, const int32_t *nbr_host,
+                     const int8_t *nbr_face_host, const int8_t *perm_host, const int8_t *bc_host,
+                     const double *bc_sign_host, bb_solver **out);
+int bb_solver_destroy(bb_solver *s);
+int bb_solver_set_state(bb_solver *s, const double *q_host);
+int bb_solver_get_state(bb_solver *s, double *q_host);
+/* device pointer of the current state ([K][F][Np] FP64) */
+int bb_solver_state_ptr(bb_solver *s, double **q_dev);
+/* du = rhs(q) of the current state, copied to du_host ([K][F][Np] FP64).  For
+ * BB_PREC_MIXED the values are the FP32 residual register widened to FP64. */
+int bb_solver_rhs(bb_solver *s, int path, int prec, double *du_host);
+/* advance nsteps LSRK45 steps of size dt */
+int bb_solver_step(bb_solver *s, int path, int prec, double dt, int nsteps);
+/* capture nsteps steps into a hipGraph, then replay it with bb_solver_graph_launch */
+int bb_solver_graph_capture(bb_solver *s, int path, int prec, double dt, int nsteps);
+int bb_solver_graph_launch(bb_solver *s);
+/* pick elements-per-workgroup of the fused stage kernel by timing every candidate on
+ * the current state (the state is restored); returns the choice */
+int bb_solver_autotune(bb_solver *s, int prec, int *elems_per_block);
+int bb_solver_set_tuning(bb_solver *s, int prec, int elems_per_block);
+/* HBM bytes one fused stage moves by the algorithmic count (state in/out, residual
+ * in/out, neighbour traces, geometry, connectivity) and the DOF count K*Np*F */
+int bb_solver_stage_bytes(bb_solver *s, int prec, double *bytes, double *dofs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BBDG_H */

@@ -1,0 +1,209 @@
+"""CPU oracle: a plain-C Float64 restatement of the reference's operators (bbref.c).
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s ``cpu_baseline`` leg may import this module, and only as the checker
+(or as the timed CPU baseline).  The product library never calls it.
+
+``build()`` compiles ``bbref.c`` with gcc into ``oracle/_build/liboracle.so``; ``lib()``
+loads it (building first only when nothing is built).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD_DIR = os.path.join(HERE, "_build")
+SO = os.path.join(BUILD_DIR, "liboracle.so")
+
+_lib = None
+
+
+def build() -> str:
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    src = os.path.join(HERE, "bbref.c")
+    tmp = SO + ".tmp"
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-std=c11", "-o", tmp, src, "-lm"]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, SO)
+    return SO
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO):
+            build()
+        L = C.CDLL(SO)
+        i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        L.orc_np.argtypes = [C.c_int]
+        L.orc_nfp.argtypes = [C.c_int]
+        L.orc_tri_offsets.argtypes = [C.c_int, i64p]
+        L.orc_tet_offsets.argtypes = [C.c_int, i64p]
+        L.orc_ij_to_linear.argtypes = [C.c_int, C.c_int, i64p]
+        L.orc_ij_to_linear.restype = C.c_int64
+        L.orc_ijk_to_linear.argtypes = [C.c_int, C.c_int, C.c_int, i64p, i64p]
+        L.orc_ijk_to_linear.restype = C.c_int64
+        L.orc_linear_to_ijkl.argtypes = [C.c_int, i32p]
+        L.orc_linear_to_ijk_2d.argtypes = [C.c_int, i32p]
+        L.orc_l_j.argtypes = [C.c_int, dp]
+        L.orc_deriv_entry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_deriv_entry.restype = C.c_double
+        L.orc_deriv_mul.argtypes = [C.c_int, C.c_int, dp, dp]
+        L.orc_deriv_mul_cols.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp]
+        L.orc_elev_entry.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_elev_entry.restype = C.c_double
+        L.orc_elevation_multiply.argtypes = [C.c_int, dp, dp]
+        L.orc_reduction_multiply.argtypes = [C.c_int, dp, dp]
+        L.orc_face_lift.argtypes = [C.c_int, dp, dp, dp, dp]
+        L.orc_lift.argtypes = [C.c_int, dp, dp, dp]
+        L.orc_lift_scratch_len.argtypes = [C.c_int]
+        L.orc_lift_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
+        L.orc_face_lift_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
+        L.orc_fmask.argtypes = [C.c_int, i32p]
+        L.orc_face_gather.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp]
+        i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+        L.orc_rhs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, i32p, i8p, dp, dp, dp, dp, dp, dp, dp, C.c_int]
+        L.orc_lsrk_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, dp, C.c_double, i32p, i8p, dp, dp, dp, dp,
+                                    dp, dp, C.c_int]
+        L.orc_lsrk_coeffs.argtypes = [dp, dp]
+        _lib = L
+    return _lib
+
+
+# ----------------------------------------------------------------- thin wrappers
+
+def Np(N):
+    return (N + 1) * (N + 2) * (N + 3) // 6
+
+
+def Nfp(N):
+    return (N + 1) * (N + 2) // 2
+
+
+def tri_offsets(N):
+    o = np.zeros(21, np.int64)
+    lib().orc_tri_offsets(N, o)
+    return o
+
+
+def tet_offsets(N):
+    o = np.zeros(21, np.int64)
+    lib().orc_tet_offsets(N, o)
+    return o
+
+
+def l_j(N):
+    o = np.zeros(20)
+    lib().orc_l_j(N, o)
+    return o
+
+
+def linear_to_ijkl(N):
+    o = np.zeros(4 * Np(N), np.int32)
+    lib().orc_linear_to_ijkl(N, o)
+    return o.reshape(-1, 4)
+
+
+def fmask(N):
+    o = np.zeros(4 * Nfp(N), np.int32)
+    lib().orc_fmask(N, o)
+    return o.reshape(4, -1)
+
+
+def deriv_dense(axis, N):
+    n = Np(N)
+    L = lib()
+    return np.array([[L.orc_deriv_entry(axis, N, m, k) for k in range(n)] for m in range(n)])
+
+
+def deriv_mul(axis, N, x):
+    """x: (ncols, Np) C-order == Julia Np x ncols column-major"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    ncols = x.size // Np(N)
+    lib().orc_deriv_mul_cols(axis, N, ncols, x.reshape(-1), out.reshape(-1))
+    return out
+
+
+def elev_dense(N):
+    L = lib()
+    return np.array([[L.orc_elev_entry(N, m, k) for k in range(Nfp(N - 1))] for m in range(Nfp(N))])
+
+
+def elevation_multiply(N, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(Nfp(N + 1))
+    lib().orc_elevation_multiply(N, x, out)
+    return out
+
+
+def reduction_multiply(N, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(Nfp(N - 1))
+    lib().orc_reduction_multiply(N, x, out)
+    return out
+
+
+def face_lift(N, x):
+    """x: (ncols, Nfp) -> (ncols, Np)"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    ncols = x.size // Nfp(N)
+    out = np.zeros((ncols, Np(N)))
+    lib().orc_face_lift_cols(N, ncols, x.reshape(-1), out.reshape(-1))
+    return out
+
+
+def lift(N, x):
+    """x: (ncols, 4Nfp) -> (ncols, Np)"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    ncols = x.size // (4 * Nfp(N))
+    out = np.zeros((ncols, Np(N)))
+    lib().orc_lift_cols(N, ncols, x.reshape(-1), out.reshape(-1))
+    return out
+
+
+def lsrk_coeffs():
+    a, b = np.zeros(5), np.zeros(5)
+    lib().orc_lsrk_coeffs(a, b)
+    return a, b
+
+
+class MeshArrays:
+    """StartUpDG-style per-face-node arrays for one mesh at degree N (oracle input)."""
+
+    def __init__(self, mesh, N, bc_sign):
+        from bbdg import mesh as M  # host setup code, not the compute path
+
+        self.N = N
+        self.K = mesh.K
+        self.mapP, self.bc = M.map_p(mesh, N)
+        self.nx, self.ny, self.nz = M.face_node_normals(mesh, N)
+        self.geo = np.ascontiguousarray(mesh.geo)
+        self.J = np.ascontiguousarray(mesh.J)
+        self.bc_sign = np.ascontiguousarray(np.asarray(bc_sign, dtype=np.float64))
+
+
+def rhs(pde, N, q_aos, ma: MeshArrays, nthreads=1, flux_kind=1):
+    """q_aos: (K, Np, F) -> du (K, Np, F)  (examples/3d_wave.jl:36-84 / 3d_maxwell.jl:180-237)"""
+    q = np.ascontiguousarray(q_aos, dtype=np.float64)
+    du = np.zeros_like(q)
+    lib().orc_rhs(pde, flux_kind, N, ma.K, q.reshape(-1), ma.mapP.reshape(-1), ma.bc.reshape(-1), ma.bc_sign,
+                  ma.nx.reshape(-1), ma.ny.reshape(-1), ma.nz.reshape(-1), ma.geo.reshape(-1), ma.J, du.reshape(-1),
+                  nthreads)
+    return du
+
+
+def lsrk_step(pde, N, q_aos, dt, ma: MeshArrays, nthreads=1, flux_kind=1):
+    """in-place LSRK45 step on q_aos (K, Np, F) float64 C-contiguous"""
+    assert q_aos.flags.c_contiguous and q_aos.dtype == np.float64
+    res = np.zeros_like(q_aos)
+    lib().orc_lsrk_step(pde, flux_kind, N, ma.K, q_aos.reshape(-1), res.reshape(-1), dt, ma.mapP.reshape(-1),
+                        ma.bc.reshape(-1), ma.bc_sign, ma.nx.reshape(-1), ma.ny.reshape(-1), ma.nz.reshape(-1),
+                        ma.geo.reshape(-1), ma.J, nthreads)
+    return q_aos
