@@ -1,0 +1,103 @@
+"""ctypes binding of libbbdg.so (the C ABI declared in include/bbdg.h).
+
+The library is built in-tree (``bernsteinbasis.jl_amd/libbbdg.so``).  There is no
+fallback: if it is missing, every compute entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(os.path.dirname(HERE), "libbbdg.so")
+
+BB_OK, BB_ERR_ARG, BB_ERR_NULL, BB_ERR_HIP, BB_ERR_UNSUPPORTED, BB_ERR_STATE = range(6)
+BB_F64, BB_F32 = 0, 1
+BB_PDE_WAVE, BB_PDE_MAXWELL = 0, 1
+BB_FLUX_REFERENCE, BB_FLUX_UPWIND = 0, 1
+BB_LIFT_PLAIN, BB_LIFT_ACC64, BB_LIFT_TWOSUM = 0, 1, 2
+BB_PREC_F64, BB_PREC_MIXED = 0, 1
+BB_PATH_PLAIN, BB_PATH_FUSED = 0, 1
+
+ERR_NAMES = {1: "BB_ERR_ARG", 2: "BB_ERR_NULL", 3: "BB_ERR_HIP", 4: "BB_ERR_UNSUPPORTED", 5: "BB_ERR_STATE"}
+
+
+class bb_layout(C.Structure):
+    _fields_ = [("elem", C.c_int64), ("node", C.c_int64), ("field", C.c_int64)]
+
+
+class BBError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+# exported symbol -> (restype, argtypes)
+vp, i64, i32, dbl = C.c_void_p, C.c_int64, C.c_int, C.c_double
+L = bb_layout
+SIGS = {
+    "bb_version": (C.c_char_p, []),
+    "bb_last_error": (C.c_char_p, [vp]),
+    "bb_device_count": (i32, [C.POINTER(C.c_int)]),
+    "bb_ctx_create": (i32, [i32, C.POINTER(vp)]),
+    "bb_ctx_destroy": (i32, [vp]),
+    "bb_ctx_stream": (i32, [vp, C.POINTER(vp)]),
+    "bb_device_info": (i32, [vp, C.c_char_p, i32, C.c_char_p, i32, C.POINTER(C.c_int)]),
+    "bb_malloc": (i32, [vp, C.c_size_t, C.POINTER(vp)]),
+    "bb_free": (i32, [vp, vp]),
+    "bb_memcpy_h2d": (i32, [vp, vp, vp, C.c_size_t]),
+    "bb_memcpy_d2h": (i32, [vp, vp, vp, C.c_size_t]),
+    "bb_memcpy_d2d": (i32, [vp, vp, vp, C.c_size_t]),
+    "bb_fill_zero": (i32, [vp, vp, C.c_size_t]),
+    "bb_synchronize": (i32, [vp]),
+    "bb_timer_start": (i32, [vp]),
+    "bb_timer_stop": (i32, [vp, C.POINTER(C.c_float)]),
+    "bb_tables": (i32, [i32, vp, vp, vp]),
+    "bb_linear_to_ijkl": (i32, [i32, vp]),
+    "bb_linear_to_ijk_2d": (i32, [i32, vp]),
+    "bb_fmask": (i32, [i32, vp]),
+    "bb_deriv_entry": (i32, [i32, i32, i32, i32, C.POINTER(C.c_double)]),
+    "bb_elevation_entry": (i32, [i32, i32, i32, C.POINTER(C.c_double)]),
+    "bb_deriv_apply": (i32, [vp, i32, i32, i32, i64, i32, vp, L, vp, L]),
+    "bb_grad_apply": (i32, [vp, i32, i32, i64, i32, vp, L, vp, vp, vp, vp, L]),
+    "bb_lift_apply": (i32, [vp, i32, i32, i64, i32, i32, vp, L, vp, L]),
+    "bb_face_lift_apply": (i32, [vp, i32, i32, i64, i32, i32, i32, vp, L, vp, L]),
+    "bb_elevate_apply": (i32, [vp, i32, i32, i64, i32, vp, L, vp, L]),
+    "bb_face_gather": (i32, [vp, i32, i32, i64, i32, vp, L, vp, L]),
+    "bb_interface_flux": (i32, [vp, i32, i32, i32, i64, vp, L, vp, vp, vp, vp, vp, vp, vp, L]),
+    "bb_solver_create": (i32, [vp, i32, i32, i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp)]),
+    "bb_solver_destroy": (i32, [vp]),
+    "bb_solver_set_state": (i32, [vp, vp]),
+    "bb_solver_get_state": (i32, [vp, vp]),
+    "bb_solver_state_ptr": (i32, [vp, C.POINTER(vp)]),
+    "bb_solver_rhs": (i32, [vp, i32, i32, vp]),
+    "bb_solver_step": (i32, [vp, i32, i32, dbl, i32]),
+    "bb_solver_graph_capture": (i32, [vp, i32, i32, dbl, i32]),
+    "bb_solver_graph_launch": (i32, [vp]),
+    "bb_solver_autotune": (i32, [vp, i32, C.POINTER(C.c_int)]),
+    "bb_solver_set_tuning": (i32, [vp, i32, i32]),
+    "bb_solver_stage_bytes": (i32, [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libbbdg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO):
+            raise RuntimeError(f"{SO} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+        h = C.CDLL(SO)
+        for name, (res, args) in SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(rc, ctx=None):
+    if rc != BB_OK:
+        msg = lib().bb_last_error(ctx).decode() if ctx else ""
+        raise BBError(rc, msg)

@@ -1,0 +1,84 @@
+// bb_common.hpp -- index algebra, context and error plumbing shared by the kernels.
+//
+// Bernstein multi-indices (i,j,k,l), i+j+k+l = N, are stored in the reference's linear
+// order: k outer, j, i inner (src/ordering.jl:37-39, docs/src/notation.md:34).  The
+// closed forms below are the 0-based restatement of src/ordering.jl:6-35:
+//   tri(n, j)     = j (2n + 3 - j) / 2            offset of row j of a degree-n triangle
+//   tet(N, k)     = Np(N) - Np(N - k)              offset of slab k
+//   lin3(N,i,j,k) = i + tri(N - k, j) + tet(N, k)  (= ijk_to_linear - 1, since
+//                                                   tri_N(j) - j k = tri_{N-k}(j))
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+#include "bbdg.h"
+
+namespace bb {
+
+__host__ __device__ constexpr int np3(int N) { return (N + 1) * (N + 2) * (N + 3) / 6; }
+__host__ __device__ constexpr int nfp2(int N) { return (N + 1) * (N + 2) / 2; }
+__host__ __device__ constexpr int tri(int n, int j) { return j * (2 * n + 3 - j) / 2; }
+__host__ __device__ constexpr int tet(int N, int k) { return np3(N) - np3(N - k); }
+__host__ __device__ constexpr int lin3(int N, int i, int j, int k) { return i + tri(N - k, j) + tet(N, k); }
+__host__ __device__ constexpr int lin2(int n, int a, int b) { return a + tri(n, b); }
+
+// Volume index of node (a,b) (c = N-a-b) of face f, faces in the order of
+// src/BernsteinLift.jl:191-217 ([f1; f2; f3; f4]):
+//   f1 (j=0): (i,k)=(a,b)   f2 (l=0): (j,k)=(a,b), i=c   f3 (i=0): (j,k)=(a,b)   f4 (k=0): (i,j)=(a,b)
+__host__ __device__ constexpr int face_vidx(int N, int f, int a, int b) {
+    return f == 0 ? lin3(N, a, 0, b) : f == 1 ? lin3(N, N - a - b, a, b) : f == 2 ? lin3(N, 0, a, b) : lin3(N, a, b, 0);
+}
+
+// l_j of src/constants.jl:10-12 (l_0 = 1 for the L0 slab); a constant expression, so the
+// division is the correctly rounded IEEE one the reference performs at run time.
+__host__ __device__ constexpr double binom_d(int n, int k) {
+    double b = 1.0;
+    for (int m = 1; m <= k; ++m) b = b * (n - k + m) / m;  // exact: partial products are integers < 2^53
+    return b;
+}
+__host__ __device__ constexpr double lj(int N, int j) {
+    return j == 0 ? 1.0 : (((j & 1) ? -1.0 : 1.0) * binom_d(N, j)) / (1.0 + j);
+}
+
+// the 6 permutations of {0,1,2} in lexicographic order (face orientation codes)
+__host__ __device__ constexpr int perm_entry(int p, int m) {
+    // rows: 012 021 102 120 201 210
+    return (p == 0) ? m : (p == 1) ? (m == 0 ? 0 : 3 - m) : (p == 2) ? (m == 0 ? 1 : m == 1 ? 0 : 2)
+         : (p == 3) ? (m == 0 ? 1 : m == 1 ? 2 : 0) : (p == 4) ? (m == 0 ? 2 : m == 1 ? 0 : 1)
+                    : (2 - m);
+}
+
+struct Lay {
+    int64_t e, n, f;
+};
+inline Lay lay(bb_layout l) { return Lay{l.elem, l.node, l.field}; }
+
+template <typename T> struct DT;
+template <> struct DT<double> { static constexpr int code = BB_F64; };
+template <> struct DT<float> { static constexpr int code = BB_F32; };
+
+}  // namespace bb
+
+struct bb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool capturing = false;
+    std::string err;
+};
+
+// error helpers (host)
+int bb_fail(bb_ctx *ctx, int code, const char *fmt, ...);
+#define BB_HIP(ctx, call)                                                                           \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return bb_fail((ctx), BB_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                                     \
+    } while (0)
+#define BB_LAUNCH_CHECK(ctx) BB_HIP(ctx, hipGetLastError())
+
+// shared argument validation
+int bb_check_common(bb_ctx *ctx, int N, int F, int64_t K, int nmin, int nmax);

@@ -1,0 +1,329 @@
+// bb_lift.hip -- the O(N^3) Bernstein lift (Chan & Warburton factorisation).
+//
+// For each face of each column:
+//   E_0 = L0 x            (L0 = (N+1)^2/2 E^T E, a 7-point stencil on the face)
+//   E_s = (E_{N-s}^{N-s+1})^T E_{s-1}     s = 1..N   (degree reduction chain)
+//   slab s of the face lift = l_s E_s      (l_s signed binomial weights)
+// src/BernsteinLift.jl:139-164 (one face) and :191-220 (four faces + permuted sum).
+//
+// GPU organisation: a block owns C columns.  The chain runs in LDS with one barrier per
+// degree; the chains of all 4 faces x C columns advance together, so a step has
+// C*4*Nfp(d) independent items.  Only the current and next chain vectors are kept
+// (two LDS buffers).  Each thread owns fixed volume nodes and, after every step, adds
+// the slab values that land on its nodes (face f4 at slab k, f1 at slab j, f2 at slab l,
+// f3 at slab i) into a private accumulator -- a deterministic gather, no atomics.
+#include "bb_dispatch.hpp"
+#include "bb_stencil.hpp"
+
+namespace bb {
+
+constexpr int kLiftThreads = 256;
+
+template <int N> struct LiftCfg {
+    static constexpr int NP = np3(N), NFP = nfp2(N);
+    static constexpr int R = ((NP + kLiftThreads - 1) / kLiftThreads) > 4 ? ((NP + kLiftThreads - 1) / kLiftThreads) : 4;
+    static constexpr int C = (kLiftThreads * R) / NP;  // columns per block (>= 1)
+};
+
+// dec[np3(d-1) + t] = (a | b << 8) for entry t of a degree-d triangle, d = 0..N
+template <int N> __device__ __forceinline__ void build_dec(uint16_t *dec) {
+    for (int t = threadIdx.x; t < np3(N); t += blockDim.x) {
+        int d = 0, u = t;
+        while (u >= nfp2(d)) { u -= nfp2(d); ++d; }
+        int a, b;
+        decode2(d, u, a, b);
+        dec[t] = (uint16_t)(a | (b << 8));
+    }
+}
+
+// per owned volume node: the slab (step) and face-node index where each face's chain lands
+struct NodeFaces {
+    int step[4];  // face order f1 f2 f3 f4
+    int idx[4];
+};
+template <int N> __device__ __forceinline__ NodeFaces node_faces(int n) {
+    int i, j, k;
+    decode3<N>(n, i, j, k);
+    const int l = N - i - j - k;
+    NodeFaces r;
+    r.step[0] = j; r.idx[0] = lin2(N - j, i, k);  // G1[(i,k,j)]
+    r.step[1] = l; r.idx[1] = lin2(N - l, j, k);  // G2[(j,k,l)]
+    r.step[2] = i; r.idx[2] = lin2(N - i, j, k);  // G3[(j,k,i)]
+    r.step[3] = k; r.idx[3] = lin2(N - k, i, j);  // face 4, written first
+    return r;
+}
+
+template <int N, typename TI, typename TC, typename TA>
+__global__ __launch_bounds__(kLiftThreads) void k_lift4(const TI *__restrict__ x, Lay lx, TI *__restrict__ out, Lay lo,
+                                                        int F, int64_t ncols) {
+    using Cfg = LiftCfg<N>;
+    constexpr int NP = Cfg::NP, NFP = Cfg::NFP, C = Cfg::C, R = Cfg::R;
+    __shared__ TC bufA[C * 4 * NFP];
+    __shared__ TC bufB[C * 4 * NFP];
+    __shared__ uint16_t dec[NP];
+    const int64_t c0 = (int64_t)blockIdx.x * C;
+    const int nc = (int)min((int64_t)C, ncols - c0);
+    build_dec<N>(dec);
+    // load face data [c][4 Nfp] -> bufB
+    for (int t = threadIdx.x; t < nc * 4 * NFP; t += kLiftThreads) {
+        const int c = t / (4 * NFP), m = t % (4 * NFP);
+        const int64_t col = c0 + c, e = col / F;
+        const int f = (int)(col % F);
+        bufB[t] = TC(x[e * lx.e + (int64_t)m * lx.n + (int64_t)f * lx.f]);
+    }
+    NodeFaces nf[R];
+    TA acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int u = threadIdx.x + r * kLiftThreads;
+        nf[r] = node_faces<N>(u < C * NP ? u % NP : 0);
+        acc[r] = TA(0);
+    }
+    __syncthreads();
+    // L0 on every face: bufB -> bufA
+    for (int t = threadIdx.x; t < nc * 4 * NFP; t += kLiftThreads) {
+        const int q = t / NFP, b = dec[np3(N - 1) + t % NFP];
+        bufA[t] = l0_apply<N, TC>(bufB + q * NFP, b & 255, b >> 8);
+    }
+    __syncthreads();
+    TC *cur = bufA, *nxt = bufB;
+    auto accumulate = [&](int s, const TC *buf, double w) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int u = threadIdx.x + r * kLiftThreads;
+            if (u < nc * NP) {
+                const int c = u / NP;
+                const TC *bc = buf + c * 4 * NFP;
+                const int order[4] = {3, 0, 1, 2};
+#pragma unroll
+                for (int o = 0; o < 4; ++o) {
+                    const int ff = order[o];
+                    if (nf[r].step[ff] == s) acc[r] += TA(w) * TA(bc[ff * NFP + nf[r].idx[ff]]);
+                }
+            }
+        }
+    };
+    accumulate(0, cur, 1.0);
+#pragma unroll
+    for (int s = 1; s <= N; ++s) {
+        const int d = N - s;  // output degree
+        const TC inv = TC(1) / TC(d + 1);
+        for (int t = threadIdx.x; t < nc * 4 * nfp2(d); t += kLiftThreads) {
+            const int q = t / nfp2(d), beta = t % nfp2(d);
+            const int b = dec[np3(d - 1) + beta];
+            nxt[q * NFP + beta] = reduce_apply<TC>(cur + q * NFP, d, b & 255, b >> 8, inv);
+        }
+        __syncthreads();
+        accumulate(s, nxt, lj(N, s));
+        TC *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int u = threadIdx.x + r * kLiftThreads;
+        if (u < nc * NP) {
+            const int64_t col = c0 + u / NP, e = col / F;
+            const int f = (int)(col % F), n = u % NP;
+            out[e * lo.e + (int64_t)n * lo.n + (int64_t)f * lo.f] = TI(acc[r]);
+        }
+    }
+}
+
+// ---- float-float (compensated) arithmetic for the chain of the conditioning study
+struct FF {
+    float hi, lo;
+};
+__device__ __forceinline__ FF ff_two_sum(float a, float b) {
+    float s = a + b;
+    float bb_ = s - a;
+    float e = (a - (s - bb_)) + (b - bb_);
+    return FF{s, e};
+}
+__device__ __forceinline__ FF ff_quick(float a, float b) {
+    float s = a + b;
+    return FF{s, b - (s - a)};
+}
+__device__ __forceinline__ FF ff_add(FF x, FF y) {
+    FF s = ff_two_sum(x.hi, y.hi);
+    float e = s.lo + x.lo + y.lo;
+    return ff_quick(s.hi, e);
+}
+__device__ __forceinline__ FF ff_mul_int(FF x, float w) {  // w a small integer, exact in float
+    float p = w * x.hi;
+    float e = fmaf(w, x.hi, -p);
+    return ff_quick(p, fmaf(w, x.lo, e));
+}
+__device__ __forceinline__ FF ff_div_int(FF x, float D) {
+    float q = x.hi / D;
+    float r = fmaf(-q, D, x.hi);  // exact remainder
+    float ql = (r + x.lo) / D;
+    return ff_quick(q, ql);
+}
+__device__ __forceinline__ FF ff_mul_d(FF x, double w) {
+    float wh = (float)w, wl = (float)(w - (double)wh);
+    float p = wh * x.hi;
+    float e = fmaf(wh, x.hi, -p);
+    e = fmaf(wh, x.lo, fmaf(wl, x.hi, e));
+    return ff_quick(p, e);
+}
+
+template <typename TC> struct Chain {
+    __device__ static TC from_f(float v) { return TC(v); }
+    __device__ static TC from_d(double v) { return TC(v); }
+    __device__ static TC reduce(const TC *x, int d, int a, int b) {
+        return reduce_apply<TC>(x, d, a, b, TC(1) / TC(d + 1));
+    }
+    __device__ static double scaled(TC v, double w) { return (double)(TC(w) * v); }
+};
+template <> struct Chain<FF> {
+    __device__ static FF from_f(float v) { return FF{v, 0.f}; }
+    __device__ static FF from_d(double v) {
+        float h = (float)v;
+        return FF{h, (float)(v - (double)h)};
+    }
+    __device__ static FF reduce(const FF *x, int d, int a, int b) {
+        const int idx = lin2(d + 1, a, b);
+        FF s = ff_mul_int(x[idx + 1], float(a + 1));
+        s = ff_add(s, ff_mul_int(x[idx + (d + 2 - b)], float(b + 1)));
+        s = ff_add(s, ff_mul_int(x[idx], float(d + 1 - a - b)));
+        return ff_div_int(s, float(d + 1));
+    }
+    __device__ static double scaled(FF v, double w) {
+        FF p = ff_mul_d(v, w);
+        return (double)p.hi + (double)p.lo;
+    }
+};
+template <> struct Chain<float> {
+    __device__ static float from_f(float v) { return v; }
+    __device__ static float from_d(double v) { return (float)v; }
+    __device__ static float reduce(const float *x, int d, int a, int b) {
+        return reduce_apply<float>(x, d, a, b, 1.0f / float(d + 1));
+    }
+    __device__ static double scaled(float v, double w) { return (double)((float)w * v); }
+};
+
+// single face: TL = L0 arithmetic type, TC = chain type, TO = output type
+template <int N, typename TI, typename TL, typename TC, typename TO>
+__global__ __launch_bounds__(kLiftThreads) void k_face_lift(const TI *__restrict__ x, Lay lx, TO *__restrict__ out,
+                                                            Lay lo, int F, int64_t ncols, int C) {
+    constexpr int NP = np3(N), NFP = nfp2(N);
+    extern __shared__ double smem_d[];
+    TC *bufA = reinterpret_cast<TC *>(smem_d);
+    TC *bufB = bufA + C * NFP;
+    TL *xin = reinterpret_cast<TL *>(bufB + C * NFP);
+    uint16_t *dec = reinterpret_cast<uint16_t *>(xin + C * NFP);
+    const int64_t c0 = (int64_t)blockIdx.x * C;
+    const int nc = (int)min((int64_t)C, ncols - c0);
+    build_dec<N>(dec);
+    for (int t = threadIdx.x; t < nc * NFP; t += kLiftThreads) {
+        const int c = t / NFP, m = t % NFP;
+        const int64_t col = c0 + c, e = col / F;
+        const int f = (int)(col % F);
+        xin[t] = TL(x[e * lx.e + (int64_t)m * lx.n + (int64_t)f * lx.f]);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nc * NFP; t += kLiftThreads) {
+        const int c = t / NFP, beta = t % NFP;
+        const int b = dec[np3(N - 1) + beta];
+        const TL v = l0_apply<N, TL>(xin + c * NFP, b & 255, b >> 8);
+        bufA[t] = Chain<TC>::from_d((double)v);
+        const int64_t col = c0 + c, e = col / F;
+        const int f = (int)(col % F);
+        out[e * lo.e + (int64_t)beta * lo.n + (int64_t)f * lo.f] = TO(v);
+    }
+    __syncthreads();
+    TC *cur = bufA, *nxt = bufB;
+#pragma unroll
+    for (int s = 1; s <= N; ++s) {
+        const int d = N - s;
+        const double w = lj(N, s);
+        for (int t = threadIdx.x; t < nc * nfp2(d); t += kLiftThreads) {
+            const int c = t / nfp2(d), beta = t % nfp2(d);
+            const int b = dec[np3(d - 1) + beta];
+            const TC v = Chain<TC>::reduce(cur + c * NFP, d, b & 255, b >> 8);
+            nxt[c * NFP + beta] = v;
+            const int64_t col = c0 + c, e = col / F;
+            const int f = (int)(col % F);
+            out[e * lo.e + (int64_t)(tet(N, s) + beta) * lo.n + (int64_t)f * lo.f] = TO(Chain<TC>::scaled(v, w));
+        }
+        __syncthreads();
+        TC *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+}
+
+template <int N, typename TI, typename TC, typename TA>
+int launch_lift4(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
+    const int64_t ncols = K * F;
+    k_lift4<N, TI, TC, TA><<<grid1d(ncols, LiftCfg<N>::C), kLiftThreads, 0, ctx->stream>>>(
+        (const TI *)x, lay(lx), (TI *)out, lay(lo), F, ncols);
+    BB_LAUNCH_CHECK(ctx);
+    return BB_OK;
+}
+
+template <int N, typename TI, typename TL, typename TC, typename TO>
+int launch_face_lift(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
+    constexpr int NFP = nfp2(N);
+    const int64_t ncols = K * F;
+    int C = 2048 / NFP;
+    if (C < 1) C = 1;
+    size_t lds = (size_t)C * NFP * (2 * sizeof(TC) + sizeof(TL)) + np3(N) * sizeof(uint16_t) + 16;
+    k_face_lift<N, TI, TL, TC, TO><<<grid1d(ncols, C), kLiftThreads, lds, ctx->stream>>>(
+        (const TI *)x, lay(lx), (TO *)out, lay(lo), F, ncols, C);
+    BB_LAUNCH_CHECK(ctx);
+    return BB_OK;
+}
+
+}  // namespace bb
+
+using namespace bb;
+
+extern "C" {
+
+int bb_lift_apply(bb_ctx *ctx, int N, int F, int64_t K, int dtype, int mode, const void *x, bb_layout lx, void *out,
+                  bb_layout lo) {
+    int rc = bb_check_common(ctx, N, F, K, 1, 20);
+    if (rc) return rc;
+    if (K && (!x || !out)) return bb_fail(ctx, BB_ERR_NULL, "null operand");
+    if (lx.elem < 0 || lx.node < 0 || lx.field < 0 || lo.elem < 0 || lo.node < 0 || lo.field < 0)
+        return bb_fail(ctx, BB_ERR_ARG, "negative stride");
+    if (K == 0) return BB_OK;
+    return dispatch_n<1, 20>(N, [&](auto nc) -> int {
+        constexpr int NN = decltype(nc)::value;
+        if (dtype == BB_F64) return launch_lift4<NN, double, double, double>(ctx, F, K, x, lx, out, lo);
+        if (dtype != BB_F32) return bb_fail(ctx, BB_ERR_ARG, "dtype %d", dtype);
+        if (mode == BB_LIFT_PLAIN) return launch_lift4<NN, float, float, float>(ctx, F, K, x, lx, out, lo);
+        if (mode == BB_LIFT_ACC64) return launch_lift4<NN, float, float, double>(ctx, F, K, x, lx, out, lo);
+        if (mode == BB_LIFT_TWOSUM)
+            return bb_fail(ctx, BB_ERR_UNSUPPORTED, "two-sum mode is implemented for the single-face lift only");
+        return bb_fail(ctx, BB_ERR_ARG, "lift mode %d", mode);
+    });
+}
+
+int bb_face_lift_apply(bb_ctx *ctx, int N, int F, int64_t K, int x_dtype, int out_dtype, int mode, const void *x,
+                       bb_layout lx, void *out, bb_layout lo) {
+    int rc = bb_check_common(ctx, N, F, K, 1, 20);
+    if (rc) return rc;
+    if (K && (!x || !out)) return bb_fail(ctx, BB_ERR_NULL, "null operand");
+    if (lx.elem < 0 || lx.node < 0 || lx.field < 0 || lo.elem < 0 || lo.node < 0 || lo.field < 0)
+        return bb_fail(ctx, BB_ERR_ARG, "negative stride");
+    if ((x_dtype != BB_F64 && x_dtype != BB_F32) || (out_dtype != BB_F64 && out_dtype != BB_F32))
+        return bb_fail(ctx, BB_ERR_ARG, "dtype %d/%d", x_dtype, out_dtype);
+    if (mode < BB_LIFT_PLAIN || mode > BB_LIFT_TWOSUM) return bb_fail(ctx, BB_ERR_ARG, "lift mode %d", mode);
+    if (K == 0) return BB_OK;
+    return dispatch_n<1, 20>(N, [&](auto nc) -> int {
+        constexpr int NN = decltype(nc)::value;
+        auto go = [&](auto to) -> int {
+            using TO = decltype(to);
+            if (x_dtype == BB_F64) return launch_face_lift<NN, double, double, double, TO>(ctx, F, K, x, lx, out, lo);
+            if (mode == BB_LIFT_PLAIN) return launch_face_lift<NN, float, float, float, TO>(ctx, F, K, x, lx, out, lo);
+            if (mode == BB_LIFT_ACC64) return launch_face_lift<NN, float, float, double, TO>(ctx, F, K, x, lx, out, lo);
+            return launch_face_lift<NN, float, float, FF, TO>(ctx, F, K, x, lx, out, lo);
+        };
+        return out_dtype == BB_F64 ? go(double(0)) : go(float(0));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,343 @@
+// bb_ops.hip -- derivative, elevation, trace-gather and interface-flux operators.
+//
+// Derivative kernels stage a block of whole columns through LDS with coalesced loads
+// (the block's elements are contiguous in both the reference's [K][Np][F] layout and the
+// solver's [K][F][Np] layout), then every thread evaluates the <= 13-point barycentric
+// stencil of its node from LDS.  They are HBM-bound: per column Np values in, Np (or
+// 3 Np) values out.
+#include "bb_dispatch.hpp"
+#include "bb_pde.hpp"
+#include "bb_stencil.hpp"
+
+namespace bb {
+
+constexpr int kDerivThreads = 256;
+constexpr int kDerivTarget = 2048;  // staged values per block
+
+// split a block-local work item t into (element, field, node): the fastest index is the
+// one with the smaller stride in memory, so consecutive threads touch consecutive words.
+__device__ __forceinline__ void split3(int t, bool field_fast, int FB, int NP, int &e, int &f, int &n) {
+    if (field_fast) {
+        f = t % FB;
+        int r = t / FB;
+        n = r % NP;
+        e = r / NP;
+    } else {
+        n = t % NP;
+        int r = t / NP;
+        f = r % FB;
+        e = r / FB;
+    }
+}
+
+template <int N, typename T, bool GRAD>
+__global__ __launch_bounds__(kDerivThreads) void k_deriv(const T *__restrict__ x, Lay lx, T *__restrict__ o0,
+                                                          T *__restrict__ o1, T *__restrict__ o2, Lay lo,
+                                                          const double *__restrict__ geo, int F, int FB, int64_t K,
+                                                          int EB, int axis) {
+    constexpr int NP = np3(N);
+    extern __shared__ double smem_d[];
+    T *sx = reinterpret_cast<T *>(smem_d);
+    int *sijk = reinterpret_cast<int *>(sx + EB * FB * NP);
+    const int64_t e0 = (int64_t)blockIdx.x * EB;
+    const int f0 = blockIdx.y * FB;
+    const int ne = (int)min((int64_t)EB, K - e0);
+    const int nf = min(FB, F - f0);
+    for (int t = threadIdx.x; t < NP; t += blockDim.x) {
+        int i, j, k;
+        decode3<N>(t, i, j, k);
+        sijk[t] = i | (j << 8) | (k << 16);
+    }
+    const bool fin = lx.f < lx.n;
+    const int nit = ne * nf * NP;
+    for (int t = threadIdx.x; t < nit; t += blockDim.x) {
+        int e, f, n;
+        split3(t, fin, nf, NP, e, f, n);
+        sx[(e * FB + f) * NP + n] = x[(e0 + e) * lx.e + (int64_t)n * lx.n + (int64_t)(f0 + f) * lx.f];
+    }
+    __syncthreads();
+    const bool fout = lo.f < lo.n;
+    for (int t = threadIdx.x; t < nit; t += blockDim.x) {
+        int e, f, n;
+        split3(t, fout, nf, NP, e, f, n);
+        const int p = sijk[n];
+        T dr, ds, dt;
+        grad_ref<N, T>(sx + (e * FB + f) * NP, n, p & 255, (p >> 8) & 255, p >> 16, dr, ds, dt);
+        const int64_t oa = (e0 + e) * lo.e + (int64_t)n * lo.n + (int64_t)(f0 + f) * lo.f;
+        if constexpr (GRAD) {
+            const double *g = geo + (e0 + e) * 9;
+            o0[oa] = T(g[0]) * dr + T(g[1]) * ds + T(g[2]) * dt;
+            o1[oa] = T(g[3]) * dr + T(g[4]) * ds + T(g[5]) * dt;
+            o2[oa] = T(g[6]) * dr + T(g[7]) * ds + T(g[8]) * dt;
+        } else {
+            o0[oa] = axis == 0 ? dr : axis == 1 ? ds : dt;
+        }
+    }
+}
+
+template <typename T, bool GRAD>
+int launch_deriv(bb_ctx *ctx, int axis, int N, int F, int64_t K, const void *x, bb_layout lx, const double *geo,
+                 void *o0, void *o1, void *o2, bb_layout lo) {
+    if (K == 0) return BB_OK;
+    return dispatch_n<1, 20>(N, [&](auto nc) -> int {
+        constexpr int NN = decltype(nc)::value;
+        constexpr int NP = np3(NN);
+        int FB = F < kDerivTarget / NP ? F : (kDerivTarget / NP > 0 ? kDerivTarget / NP : 1);
+        int EB = kDerivTarget / (FB * NP);
+        if (EB < 1) EB = 1;
+        dim3 grid(grid1d(K, EB), (unsigned)((F + FB - 1) / FB));
+        size_t lds = (size_t)EB * FB * NP * sizeof(T) + NP * sizeof(int);
+        k_deriv<NN, T, GRAD><<<grid, kDerivThreads, lds, ctx->stream>>>(
+            (const T *)x, lay(lx), (T *)o0, (T *)o1, (T *)o2, lay(lo), geo, F, FB, K, EB, axis);
+        BB_LAUNCH_CHECK(ctx);
+        return BB_OK;
+    });
+}
+
+// ElevationMatrix{N} * x  (degree N-1 -> N), elevation_multiply! order of operations
+template <typename T>
+__global__ void k_elevate(int N, const T *__restrict__ x, Lay lx, T *__restrict__ out, Lay lo, int F, int64_t K) {
+    const int NF = nfp2(N);
+    const int64_t total = K * F * NF;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int m = (int)(t % NF);
+        const int64_t c = t / NF;
+        const int64_t e = c / F;
+        const int f = (int)(c % F);
+        int a, b;
+        decode2(N, m, a, b);
+        const int cc = N - a - b;
+        const T *xc = x + e * lx.e + (int64_t)f * lx.f;
+        T v = T(0);
+        if (a > 0) v = T(a) * xc[(int64_t)lin2(N - 1, a - 1, b) * lx.n];
+        if (b > 0) v = fma(T(b), xc[(int64_t)lin2(N - 1, a, b - 1) * lx.n], v);
+        if (cc > 0) v = fma(T(cc), xc[(int64_t)lin2(N - 1, a, b) * lx.n], v);
+        out[e * lo.e + (int64_t)m * lo.n + (int64_t)f * lo.f] = v / T(N);
+    }
+}
+
+// uM .= q[Fmask, :]
+template <typename T>
+__global__ void k_gather(int N, const T *__restrict__ q, Lay lq, T *__restrict__ uM, Lay lm, int F, int64_t K) {
+    const int NF = nfp2(N);
+    const int64_t total = K * F * 4 * NF;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int m = (int)(t % (4 * NF));
+        const int64_t c = t / (4 * NF);
+        const int64_t e = c / F;
+        const int f = (int)(c % F);
+        int a, b;
+        decode2(N, m % NF, a, b);
+        const int v = face_vidx(N, m / NF, a, b);
+        uM[e * lm.e + (int64_t)m * lm.n + (int64_t)f * lm.f] = q[e * lq.e + (int64_t)v * lq.n + (int64_t)f * lq.f];
+    }
+}
+
+struct Sign6 {
+    double s[6];
+};
+
+template <int PDE>
+__global__ void k_interface_flux(int kind, int N, const double *__restrict__ uM, Lay lm, const int32_t *__restrict__ mapP,
+                                 const int8_t *__restrict__ bc, Sign6 sg, const double *__restrict__ nx,
+                                 const double *__restrict__ ny, const double *__restrict__ nz, double *__restrict__ flux,
+                                 Lay lf, int64_t K) {
+    constexpr int F = Pde<PDE>::F;
+    const int M4 = 4 * nfp2(N);
+    const int64_t total = K * M4;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = idx / M4;
+        const int m = (int)(idx % M4);
+        double um[F], d[F], o[F];
+        const double *pm = uM + e * lm.e + (int64_t)m * lm.n;
+#pragma unroll
+        for (int f = 0; f < F; ++f) um[f] = pm[(int64_t)f * lm.f];
+        if (bc && bc[idx]) {
+#pragma unroll
+            for (int f = 0; f < F; ++f) d[f] = sg.s[f] * um[f] - um[f];
+        } else {
+            const int64_t p = mapP[idx];
+            const double *pp = uM + (p / M4) * lm.e + (int64_t)(p % M4) * lm.n;
+#pragma unroll
+            for (int f = 0; f < F; ++f) d[f] = pp[(int64_t)f * lm.f] - um[f];
+        }
+        Pde<PDE>::flux(kind, d, nx[idx], ny[idx], nz[idx], o);
+        double *po = flux + e * lf.e + (int64_t)m * lf.n;
+#pragma unroll
+        for (int f = 0; f < F; ++f) po[(int64_t)f * lf.f] = o[f];
+    }
+}
+
+// ---- plain-path helpers (solver layout [K][F][Np], FP64)
+
+template <int PDE>
+__global__ void k_pde_fdir(int dir, int NP, const double *__restrict__ q, double *__restrict__ out, int64_t K) {
+    constexpr int F = Pde<PDE>::F;
+    const int64_t total = K * NP;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = t / NP;
+        const int n = (int)(t % NP);
+        double qv[F], o[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) qv[f] = q[(e * F + f) * NP + n];
+        Pde<PDE>::fdir(dir, qv, o);
+#pragma unroll
+        for (int f = 0; f < F; ++f) out[(e * F + f) * NP + n] = o[f];
+    }
+}
+
+struct Ptr9 {
+    const double *d[9];
+};
+
+// du -= (or +=) rxJ*dfxdr + sxJ*dfxds + ... + tzJ*dfzdt; du /= J  (examples/3d_wave.jl:75-83)
+__global__ void k_plain_combine(bool plus, int FNP, double *__restrict__ du, Ptr9 d, const double *__restrict__ geo,
+                                const double *__restrict__ J, int64_t K) {
+    const int64_t total = K * FNP;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = t / FNP;
+        const double *g = geo + e * 9;
+        double s = g[0] * d.d[0][t] + g[1] * d.d[1][t] + g[2] * d.d[2][t] + g[3] * d.d[3][t] + g[4] * d.d[4][t] +
+                   g[5] * d.d[5][t] + g[6] * d.d[6][t] + g[7] * d.d[7][t] + g[8] * d.d[8][t];
+        double v = plus ? du[t] + s : du[t] - s;
+        du[t] = v / J[e];
+    }
+}
+
+__global__ void k_lsrk_update(int64_t n, double *__restrict__ q, double *__restrict__ res, const double *__restrict__ du,
+                              double a, double b, double dt, int first) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        double r = first ? dt * du[t] : a * res[t] + dt * du[t];
+        res[t] = r;
+        q[t] = q[t] + b * r;
+    }
+}
+
+inline unsigned grid_cap(int64_t work) {
+    int64_t g = (work + 255) / 256;
+    return (unsigned)(g > 262144 ? 262144 : (g < 1 ? 1 : g));
+}
+
+}  // namespace bb
+
+using namespace bb;
+
+static int check_layout(bb_ctx *ctx, bb_layout l, const char *what) {
+    if (l.elem < 0 || l.node < 0 || l.field < 0)
+        return bb_fail(ctx, BB_ERR_ARG, "negative stride in layout of %s", what);
+    return BB_OK;
+}
+
+extern "C" {
+
+int bb_deriv_apply(bb_ctx *ctx, int axis, int N, int F, int64_t K, int dtype, const void *x, bb_layout lx, void *out,
+                   bb_layout lo) {
+    int rc = bb_check_common(ctx, N, F, K, 1, 20);
+    if (rc) return rc;
+    if (axis < 0 || axis > 2) return bb_fail(ctx, BB_ERR_ARG, "axis %d not in {0,1,2}", axis);
+    if (K && (!x || !out)) return bb_fail(ctx, BB_ERR_NULL, "null operand");
+    if ((rc = check_layout(ctx, lx, "x")) || (rc = check_layout(ctx, lo, "out"))) return rc;
+    if (dtype == BB_F64) return launch_deriv<double, false>(ctx, axis, N, F, K, x, lx, nullptr, out, out, out, lo);
+    if (dtype == BB_F32) return launch_deriv<float, false>(ctx, axis, N, F, K, x, lx, nullptr, out, out, out, lo);
+    return bb_fail(ctx, BB_ERR_ARG, "dtype %d", dtype);
+}
+
+int bb_grad_apply(bb_ctx *ctx, int N, int F, int64_t K, int dtype, const void *x, bb_layout lx, const double *geo,
+                  void *dx, void *dy, void *dz, bb_layout lo) {
+    int rc = bb_check_common(ctx, N, F, K, 1, 20);
+    if (rc) return rc;
+    if (K && (!x || !geo || !dx || !dy || !dz)) return bb_fail(ctx, BB_ERR_NULL, "null operand");
+    if ((rc = check_layout(ctx, lx, "x")) || (rc = check_layout(ctx, lo, "out"))) return rc;
+    if (dtype == BB_F64) return launch_deriv<double, true>(ctx, 0, N, F, K, x, lx, geo, dx, dy, dz, lo);
+    if (dtype == BB_F32) return launch_deriv<float, true>(ctx, 0, N, F, K, x, lx, geo, dx, dy, dz, lo);
+    return bb_fail(ctx, BB_ERR_ARG, "dtype %d", dtype);
+}
+
+int bb_elevate_apply(bb_ctx *ctx, int N, int F, int64_t K, int dtype, const void *x, bb_layout lx, void *out,
+                     bb_layout lo) {
+    int rc = bb_check_common(ctx, N, F, K, 1, 20);
+    if (rc) return rc;
+    if (K && (!x || !out)) return bb_fail(ctx, BB_ERR_NULL, "null operand");
+    if ((rc = check_layout(ctx, lx, "x")) || (rc = check_layout(ctx, lo, "out"))) return rc;
+    if (K == 0) return BB_OK;
+    unsigned g = grid_cap(K * F * nfp2(N));
+    if (dtype == BB_F64)
+        k_elevate<double><<<g, 256, 0, ctx->stream>>>(N, (const double *)x, lay(lx), (double *)out, lay(lo), F, K);
+    else if (dtype == BB_F32)
+        k_elevate<float><<<g, 256, 0, ctx->stream>>>(N, (const float *)x, lay(lx), (float *)out, lay(lo), F, K);
+    else
+        return bb_fail(ctx, BB_ERR_ARG, "dtype %d", dtype);
+    BB_LAUNCH_CHECK(ctx);
+    return BB_OK;
+}
+
+int bb_face_gather(bb_ctx *ctx, int N, int F, int64_t K, int dtype, const void *q, bb_layout lq, void *uM,
+                   bb_layout lm) {
+    int rc = bb_check_common(ctx, N, F, K, 1, 20);
+    if (rc) return rc;
+    if (K && (!q || !uM)) return bb_fail(ctx, BB_ERR_NULL, "null operand");
+    if ((rc = check_layout(ctx, lq, "q")) || (rc = check_layout(ctx, lm, "uM"))) return rc;
+    if (K == 0) return BB_OK;
+    unsigned g = grid_cap(K * F * 4 * nfp2(N));
+    if (dtype == BB_F64)
+        k_gather<double><<<g, 256, 0, ctx->stream>>>(N, (const double *)q, lay(lq), (double *)uM, lay(lm), F, K);
+    else if (dtype == BB_F32)
+        k_gather<float><<<g, 256, 0, ctx->stream>>>(N, (const float *)q, lay(lq), (float *)uM, lay(lm), F, K);
+    else
+        return bb_fail(ctx, BB_ERR_ARG, "dtype %d", dtype);
+    BB_LAUNCH_CHECK(ctx);
+    return BB_OK;
+}
+
+int bb_interface_flux(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *uM, bb_layout lm,
+                      const int32_t *mapP, const int8_t *bc, const double *bc_sign_host, const double *nxJ,
+                      const double *nyJ, const double *nzJ, double *flux, bb_layout lf) {
+    if (pde != BB_PDE_WAVE && pde != BB_PDE_MAXWELL) return bb_fail(ctx, BB_ERR_ARG, "pde %d", pde);
+    int F = pde == BB_PDE_WAVE ? 4 : 6;
+    int rc = bb_check_common(ctx, N, F, K, 1, 20);
+    if (rc) return rc;
+    if (flux_kind != BB_FLUX_REFERENCE && flux_kind != BB_FLUX_UPWIND) return bb_fail(ctx, BB_ERR_ARG, "flux %d", flux_kind);
+    if (K && (!uM || !mapP || !nxJ || !nyJ || !nzJ || !flux)) return bb_fail(ctx, BB_ERR_NULL, "null operand");
+    if (bc && !bc_sign_host) return bb_fail(ctx, BB_ERR_NULL, "bc given without bc_sign");
+    if (K == 0) return BB_OK;
+    Sign6 sg{};
+    for (int f = 0; f < F; ++f) sg.s[f] = bc ? bc_sign_host[f] : 1.0;
+    unsigned g = grid_cap(K * 4 * nfp2(N));
+    if (pde == BB_PDE_WAVE)
+        k_interface_flux<BB_PDE_WAVE><<<g, 256, 0, ctx->stream>>>(flux_kind, N, uM, lay(lm), mapP, bc, sg, nxJ, nyJ, nzJ,
+                                                                  flux, lay(lf), K);
+    else
+        k_interface_flux<BB_PDE_MAXWELL><<<g, 256, 0, ctx->stream>>>(flux_kind, N, uM, lay(lm), mapP, bc, sg, nxJ, nyJ,
+                                                                     nzJ, flux, lay(lf), K);
+    BB_LAUNCH_CHECK(ctx);
+    return BB_OK;
+}
+
+}  // extern "C"
+
+int bb_pde_fdir(bb_ctx *ctx, int pde, int dir, int N, int64_t K, const double *q, double *out) {
+    int NP = np3(N);
+    unsigned g = grid_cap(K * NP);
+    if (pde == BB_PDE_WAVE) k_pde_fdir<BB_PDE_WAVE><<<g, 256, 0, ctx->stream>>>(dir, NP, q, out, K);
+    else k_pde_fdir<BB_PDE_MAXWELL><<<g, 256, 0, ctx->stream>>>(dir, NP, q, out, K);
+    BB_LAUNCH_CHECK(ctx);
+    return BB_OK;
+}
+
+int bb_plain_combine(bb_ctx *ctx, int pde, int N, int64_t K, double *du, double *const d[9], const double *geo,
+                     const double *J) {
+    int F = pde == BB_PDE_WAVE ? 4 : 6;
+    int FNP = F * np3(N);
+    Ptr9 p;
+    for (int m = 0; m < 9; ++m) p.d[m] = d[m];
+    k_plain_combine<<<grid_cap(K * FNP), 256, 0, ctx->stream>>>(pde == BB_PDE_MAXWELL, FNP, du, p, geo, J, K);
+    BB_LAUNCH_CHECK(ctx);
+    return BB_OK;
+}
+
+int bb_lsrk_update(bb_ctx *ctx, int64_t n, double *q, double *res, const double *du, double a, double b, double dt,
+                   int first) {
+    k_lsrk_update<<<grid_cap(n), 256, 0, ctx->stream>>>(n, q, res, du, a, b, dt, first);
+    BB_LAUNCH_CHECK(ctx);
+    return BB_OK;
+}

@@ -1,0 +1,461 @@
+// bb_solver.hip -- device-resident mesh + state, the two RHS paths, LSRK45 stepping,
+// hipGraph capture of whole time steps, and the elements-per-workgroup autotuner.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "bb_dispatch.hpp"
+#include "bb_stage.hpp"
+
+using namespace bb;
+
+// Five-stage fourth-order 2N-storage RK (Carpenter & Kennedy, NASA TM-109112, 1994)
+static const double kA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                             -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                             1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                             2277821191437.0 / 14882151754819.0};
+
+struct bb_solver {
+    bb_ctx *ctx = nullptr;
+    int pde = 0, F = 0, N = 0, flux_kind = 1, NP = 0, NFP = 0;
+    int64_t K = 0;
+    double sign[6] = {1, 1, 1, 1, 1, 1};
+    // mesh
+    double *geo = nullptr, *J = nullptr, *nJ = nullptr;
+    int2 *conn = nullptr;
+    std::vector<int2> conn_host;
+    std::vector<double> nJ_host;
+    // plain-path mesh arrays (StartUpDG layout) and scratch, allocated on first use
+    int32_t *mapP = nullptr;
+    int8_t *bcn = nullptr;
+    double *nx = nullptr, *ny = nullptr, *nz = nullptr;
+    double *uM = nullptr, *flux = nullptr, *fu[3] = {nullptr, nullptr, nullptr}, *d9[9] = {}, *du = nullptr;
+    // state
+    double *q[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *res64 = nullptr;
+    float *res32 = nullptr;
+    int eb[2] = {0, 0};  // fused elements per block, per precision (0 = not chosen)
+    // graphs
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int g_path = -1, g_start = 0, g_flip = 0;
+};
+
+static StageLaunchFn stage_fn(int pde, int prec) {
+    if (pde == BB_PDE_WAVE) return prec == BB_PREC_MIXED ? stage_launch_wave_mixed : stage_launch_wave_f64;
+    return prec == BB_PREC_MIXED ? stage_launch_maxwell_mixed : stage_launch_maxwell_f64;
+}
+
+static size_t state_words(const bb_solver *s) { return (size_t)s->K * s->F * s->NP; }
+
+static int ensure_res(bb_solver *s, int prec) {
+    bb_ctx *ctx = s->ctx;
+    size_t n = state_words(s);
+    if (prec == BB_PREC_F64 && !s->res64) {
+        BB_HIP(ctx, hipMalloc(&s->res64, n * sizeof(double)));
+        int rc = bb_fill_zero(ctx, s->res64, n * sizeof(double));
+        if (rc) return rc;
+    }
+    if (prec == BB_PREC_MIXED && !s->res32) {
+        BB_HIP(ctx, hipMalloc(&s->res32, n * sizeof(float)));
+        int rc = bb_fill_zero(ctx, s->res32, n * sizeof(float));
+        if (rc) return rc;
+    }
+    return BB_OK;
+}
+
+// default elements per block: the fewest elements that give >= 192 owned threads
+static int default_eb(const bb_solver *s, int prec) {
+    int best = 0;
+    for (int e : kStageEB) {
+        StageArgs a{};
+        (void)a;
+        if (e * s->NP > 1024) break;
+        best = e;
+        if (e * s->NP >= 192) break;
+    }
+    (void)prec;
+    return best ? best : 1;
+}
+
+static int launch_stage(bb_solver *s, int prec, int eb, const StageArgs &a) {
+    int rc = stage_fn(s->pde, prec)(s->N, eb, a, s->ctx->stream);
+    if (rc == BB_ERR_UNSUPPORTED)
+        return bb_fail(s->ctx, rc, "no fused kernel for N=%d with %d elements per block", s->N, eb);
+    if (rc) return bb_fail(s->ctx, rc, "fused stage launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return BB_OK;
+}
+
+static StageArgs stage_args(bb_solver *s, int prec) {
+    StageArgs a{};
+    a.geo = s->geo;
+    a.J = s->J;
+    a.nJ = s->nJ;
+    a.conn = s->conn;
+    a.K = s->K;
+    a.flux_kind = s->flux_kind;
+    for (int f = 0; f < 6; ++f) a.sign[f] = s->sign[f];
+    a.res = prec == BB_PREC_MIXED ? (void *)s->res32 : (void *)s->res64;
+    return a;
+}
+
+// ---- plain path: one kernel per operator of the reference RHS
+static int ensure_plain(bb_solver *s) {
+    if (s->mapP) return BB_OK;
+    bb_ctx *ctx = s->ctx;
+    const int NFP = s->NFP, N = s->N;
+    const int64_t M = s->K * 4 * NFP;
+    std::vector<int32_t> mp(M);
+    std::vector<int8_t> bc(M);
+    std::vector<double> nx(M), ny(M), nz(M);
+    std::vector<int> ab(NFP * 2);
+    for (int m = 0; m < NFP; ++m) {
+        int a, b;
+        decode2(N, m, a, b);
+        ab[2 * m] = a;
+        ab[2 * m + 1] = b;
+    }
+    for (int64_t e = 0; e < s->K; ++e)
+        for (int f = 0; f < 4; ++f) {
+            const int2 cn = s->conn_host[e * 4 + f];
+            const int code = (cn.y >> 5) & 7, nf = cn.y & 3, p = (cn.y >> 2) & 7;
+            for (int m = 0; m < NFP; ++m) {
+                const int64_t idx = (e * 4 + f) * NFP + m;
+                const int abc[3] = {ab[2 * m], ab[2 * m + 1], N - ab[2 * m] - ab[2 * m + 1]};
+                if (code) {
+                    mp[idx] = (int32_t)idx;
+                } else {
+                    const int a2 = abc[perm_entry(p, 0)], b2 = abc[perm_entry(p, 1)];
+                    mp[idx] = (int32_t)(((int64_t)cn.x * 4 + nf) * NFP + lin2(N, a2, b2));
+                }
+                bc[idx] = (int8_t)code;
+                nx[idx] = s->nJ_host[(e * 4 + f) * 3 + 0];
+                ny[idx] = s->nJ_host[(e * 4 + f) * 3 + 1];
+                nz[idx] = s->nJ_host[(e * 4 + f) * 3 + 2];
+            }
+        }
+    const size_t fw = (size_t)M * s->F, vw = state_words(s);
+    BB_HIP(ctx, hipMalloc(&s->mapP, M * sizeof(int32_t)));
+    BB_HIP(ctx, hipMalloc(&s->bcn, M));
+    BB_HIP(ctx, hipMalloc(&s->nx, M * sizeof(double)));
+    BB_HIP(ctx, hipMalloc(&s->ny, M * sizeof(double)));
+    BB_HIP(ctx, hipMalloc(&s->nz, M * sizeof(double)));
+    BB_HIP(ctx, hipMemcpy(s->mapP, mp.data(), M * sizeof(int32_t), hipMemcpyHostToDevice));
+    BB_HIP(ctx, hipMemcpy(s->bcn, bc.data(), M, hipMemcpyHostToDevice));
+    BB_HIP(ctx, hipMemcpy(s->nx, nx.data(), M * sizeof(double), hipMemcpyHostToDevice));
+    BB_HIP(ctx, hipMemcpy(s->ny, ny.data(), M * sizeof(double), hipMemcpyHostToDevice));
+    BB_HIP(ctx, hipMemcpy(s->nz, nz.data(), M * sizeof(double), hipMemcpyHostToDevice));
+    BB_HIP(ctx, hipMalloc(&s->uM, fw * sizeof(double)));
+    BB_HIP(ctx, hipMalloc(&s->flux, fw * sizeof(double)));
+    for (auto &p : s->fu) BB_HIP(ctx, hipMalloc(&p, vw * sizeof(double)));
+    for (auto &p : s->d9) BB_HIP(ctx, hipMalloc(&p, vw * sizeof(double)));
+    BB_HIP(ctx, hipMalloc(&s->du, vw * sizeof(double)));
+    return BB_OK;
+}
+
+static int plain_rhs(bb_solver *s, const double *q) {
+    bb_ctx *ctx = s->ctx;
+    const int F = s->F, NP = s->NP, NFP = s->NFP, N = s->N;
+    const int64_t K = s->K;
+    const bb_layout lv{(int64_t)F * NP, 1, NP};        // [K][F][Np]
+    const bb_layout lf{(int64_t)F * 4 * NFP, 1, 4 * NFP};  // [K][F][4Nfp]
+    int rc;
+    if ((rc = bb_face_gather(ctx, N, F, K, BB_F64, q, lv, s->uM, lf))) return rc;
+    if ((rc = bb_interface_flux(ctx, s->pde, s->flux_kind, N, K, s->uM, lf, s->mapP, s->bcn, s->sign, s->nx, s->ny,
+                                s->nz, s->flux, lf)))
+        return rc;
+    for (int dir = 0; dir < 3; ++dir)
+        if ((rc = bb_pde_fdir(ctx, s->pde, dir, N, K, q, s->fu[dir]))) return rc;
+    for (int c = 0; c < 3; ++c)
+        for (int ax = 0; ax < 3; ++ax)
+            if ((rc = bb_deriv_apply(ctx, ax, N, F, K, BB_F64, s->fu[c], lv, s->d9[3 * c + ax], lv))) return rc;
+    if ((rc = bb_lift_apply(ctx, N, F, K, BB_F64, BB_LIFT_PLAIN, s->flux, lf, s->du, lv))) return rc;
+    return bb_plain_combine(ctx, s->pde, N, K, s->du, s->d9, s->geo, s->J);
+}
+
+// enqueue nsteps steps (no synchronisation; capturable)
+static int enqueue_steps(bb_solver *s, int path, int prec, double dt, int nsteps, int *cur) {
+    int rc;
+    const int64_t nw = (int64_t)state_words(s);
+    for (int st = 0; st < nsteps; ++st)
+        for (int stage = 0; stage < 5; ++stage) {
+            if (path == BB_PATH_PLAIN) {
+                if ((rc = plain_rhs(s, s->q[*cur]))) return rc;
+                if ((rc = bb_lsrk_update(s->ctx, nw, s->q[*cur], s->res64, s->du, kA[stage], kB[stage], dt, stage == 0)))
+                    return rc;
+            } else {
+                StageArgs a = stage_args(s, prec);
+                a.q_in = s->q[*cur];
+                a.q_out = s->q[*cur ^ 1];
+                a.ca = kA[stage];
+                a.cb = kB[stage];
+                a.dt = dt;
+                a.first = stage == 0;
+                if ((rc = launch_stage(s, prec, s->eb[prec], a))) return rc;
+                *cur ^= 1;
+            }
+        }
+    return BB_OK;
+}
+
+static int check_path(bb_solver *s, int path, int prec) {
+    if (path != BB_PATH_PLAIN && path != BB_PATH_FUSED) return bb_fail(s->ctx, BB_ERR_ARG, "path %d", path);
+    if (prec != BB_PREC_F64 && prec != BB_PREC_MIXED) return bb_fail(s->ctx, BB_ERR_ARG, "prec %d", prec);
+    if (path == BB_PATH_PLAIN && prec != BB_PREC_F64)
+        return bb_fail(s->ctx, BB_ERR_UNSUPPORTED, "the plain path is FP64 only");
+    int rc;
+    if ((rc = ensure_res(s, path == BB_PATH_PLAIN ? BB_PREC_F64 : prec))) return rc;
+    if (path == BB_PATH_PLAIN) return ensure_plain(s);
+    if (!s->eb[prec]) s->eb[prec] = default_eb(s, prec);
+    return BB_OK;
+}
+
+static void drop_graphs(bb_solver *s) {
+    for (auto &g : s->gexec)
+        if (g) {
+            hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    s->g_path = -1;
+}
+
+extern "C" {
+
+int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *geo_host,
+                     const double *J_host, const double *nJ_host, const int32_t *nbr_host,
+                     const int8_t *nbr_face_host, const int8_t *perm_host, const int8_t *bc_host,
+                     const double *bc_sign_host, bb_solver **out) {
+    if (!ctx || !out) return BB_ERR_NULL;
+    *out = nullptr;
+    if (pde != BB_PDE_WAVE && pde != BB_PDE_MAXWELL) return bb_fail(ctx, BB_ERR_ARG, "pde %d", pde);
+    if (flux_kind != BB_FLUX_REFERENCE && flux_kind != BB_FLUX_UPWIND) return bb_fail(ctx, BB_ERR_ARG, "flux %d", flux_kind);
+    const int F = pde == BB_PDE_WAVE ? 4 : 6;
+    int rc = bb_check_common(ctx, N, F, K, 1, kStageMaxN);
+    if (rc) return rc;
+    if (K < 1) return bb_fail(ctx, BB_ERR_ARG, "K must be >= 1");
+    if (!geo_host || !J_host || !nJ_host || !nbr_host || !nbr_face_host || !perm_host || !bc_host)
+        return bb_fail(ctx, BB_ERR_NULL, "null mesh array");
+    std::vector<int2> conn(K * 4);
+    for (int64_t t = 0; t < K * 4; ++t) {
+        const int nb = nbr_host[t], nf = nbr_face_host[t], p = perm_host[t], bc = bc_host[t];
+        if (nb < 0 || nb >= K || nf < 0 || nf > 3 || p < 0 || p > 5 || bc < 0 || bc > 1)
+            return bb_fail(ctx, BB_ERR_ARG, "bad connectivity at element %lld face %d", (long long)(t / 4), (int)(t % 4));
+        conn[t] = make_int2(nb, nf | (p << 2) | (bc << 5));
+    }
+    for (int64_t e = 0; e < K; ++e)
+        if (!(J_host[e] > 0.0)) return bb_fail(ctx, BB_ERR_ARG, "non-positive Jacobian at element %lld", (long long)e);
+    bb_solver *s = new bb_solver();
+    s->ctx = ctx;
+    s->pde = pde;
+    s->F = F;
+    s->N = N;
+    s->K = K;
+    s->flux_kind = flux_kind;
+    s->NP = np3(N);
+    s->NFP = nfp2(N);
+    if (bc_sign_host)
+        for (int f = 0; f < F; ++f) s->sign[f] = bc_sign_host[f];
+    s->conn_host = conn;
+    s->nJ_host.assign(nJ_host, nJ_host + K * 12);
+    auto fail = [&](hipError_t e) {
+        int r = bb_fail(ctx, BB_ERR_HIP, "solver allocation: %s", hipGetErrorString(e));
+        bb_solver_destroy(s);
+        return r;
+    };
+    hipError_t e;
+    const size_t vw = state_words(s);
+    if ((e = hipMalloc(&s->geo, K * 9 * sizeof(double))) || (e = hipMalloc(&s->J, K * sizeof(double))) ||
+        (e = hipMalloc(&s->nJ, K * 12 * sizeof(double))) || (e = hipMalloc(&s->conn, K * 4 * sizeof(int2))) ||
+        (e = hipMalloc(&s->q[0], vw * sizeof(double))) || (e = hipMalloc(&s->q[1], vw * sizeof(double))))
+        return fail(e);
+    if ((e = hipMemcpy(s->geo, geo_host, K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->J, J_host, K * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->nJ, nJ_host, K * 12 * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->conn, conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)))
+        return fail(e);
+    if ((rc = bb_fill_zero(ctx, s->q[0], vw * sizeof(double))) || (rc = bb_fill_zero(ctx, s->q[1], vw * sizeof(double)))) {
+        bb_solver_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return BB_OK;
+}
+
+int bb_solver_destroy(bb_solver *s) {
+    if (!s) return BB_ERR_NULL;
+    hipStreamSynchronize(s->ctx->stream);
+    drop_graphs(s);
+    void *ptrs[] = {s->geo, s->J, s->nJ, s->conn, s->mapP, s->bcn, s->nx, s->ny, s->nz, s->uM, s->flux, s->du,
+                    s->q[0], s->q[1], s->res64, s->res32};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (auto p : s->fu)
+        if (p) hipFree(p);
+    for (auto p : s->d9)
+        if (p) hipFree(p);
+    delete s;
+    return BB_OK;
+}
+
+int bb_solver_set_state(bb_solver *s, const double *q_host) {
+    if (!s || !q_host) return BB_ERR_NULL;
+    BB_HIP(s->ctx, hipMemcpyAsync(s->q[s->cur], q_host, state_words(s) * sizeof(double), hipMemcpyHostToDevice,
+                                  s->ctx->stream));
+    BB_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    return BB_OK;
+}
+
+int bb_solver_get_state(bb_solver *s, double *q_host) {
+    if (!s || !q_host) return BB_ERR_NULL;
+    BB_HIP(s->ctx, hipMemcpyAsync(q_host, s->q[s->cur], state_words(s) * sizeof(double), hipMemcpyDeviceToHost,
+                                  s->ctx->stream));
+    BB_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    return BB_OK;
+}
+
+int bb_solver_state_ptr(bb_solver *s, double **q_dev) {
+    if (!s || !q_dev) return BB_ERR_NULL;
+    *q_dev = s->q[s->cur];
+    return BB_OK;
+}
+
+int bb_solver_rhs(bb_solver *s, int path, int prec, double *du_host) {
+    if (!s || !du_host) return BB_ERR_NULL;
+    int rc = check_path(s, path, prec);
+    if (rc) return rc;
+    bb_ctx *ctx = s->ctx;
+    const size_t vw = state_words(s);
+    if (path == BB_PATH_PLAIN) {
+        if ((rc = plain_rhs(s, s->q[s->cur]))) return rc;
+        BB_HIP(ctx, hipMemcpyAsync(du_host, s->du, vw * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        BB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return BB_OK;
+    }
+    StageArgs a = stage_args(s, prec);
+    a.q_in = s->q[s->cur];
+    a.q_out = nullptr;
+    a.rhs_only = 1;
+    a.first = 1;
+    a.dt = 1.0;
+    if ((rc = launch_stage(s, prec, s->eb[prec], a))) return rc;
+    if (prec == BB_PREC_F64) {
+        BB_HIP(ctx, hipMemcpyAsync(du_host, s->res64, vw * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        BB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        std::vector<float> tmp(vw);
+        BB_HIP(ctx, hipMemcpyAsync(tmp.data(), s->res32, vw * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t t = 0; t < vw; ++t) du_host[t] = tmp[t];
+    }
+    return BB_OK;
+}
+
+int bb_solver_step(bb_solver *s, int path, int prec, double dt, int nsteps) {
+    if (!s) return BB_ERR_NULL;
+    if (nsteps < 0) return bb_fail(s->ctx, BB_ERR_ARG, "nsteps %d", nsteps);
+    int rc = check_path(s, path, prec);
+    if (rc) return rc;
+    return enqueue_steps(s, path, prec, dt, nsteps, &s->cur);
+}
+
+int bb_solver_graph_capture(bb_solver *s, int path, int prec, double dt, int nsteps) {
+    if (!s) return BB_ERR_NULL;
+    if (nsteps < 1) return bb_fail(s->ctx, BB_ERR_ARG, "nsteps %d", nsteps);
+    int rc = check_path(s, path, prec);
+    if (rc) return rc;
+    bb_ctx *ctx = s->ctx;
+    BB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graphs(s);
+    // a fused step flips the ping-pong buffers once per stage; with an odd number of
+    // stages per graph, capture one graph per starting buffer
+    const int flip = path == BB_PATH_FUSED ? (5 * nsteps) & 1 : 0;
+    for (int g = 0; g < (flip ? 2 : 1); ++g) {
+        int start = s->cur ^ g, c = start;
+        hipGraph_t graph = nullptr;
+        BB_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        ctx->capturing = true;
+        rc = enqueue_steps(s, path, prec, dt, nsteps, &c);
+        ctx->capturing = false;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        if (rc) {
+            if (graph) hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return bb_fail(ctx, BB_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&s->gexec[start], graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return bb_fail(ctx, BB_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    s->g_path = path;
+    s->g_flip = flip;
+    return BB_OK;
+}
+
+int bb_solver_graph_launch(bb_solver *s) {
+    if (!s) return BB_ERR_NULL;
+    if (s->g_path < 0 || !s->gexec[s->cur]) return bb_fail(s->ctx, BB_ERR_STATE, "no graph captured for this buffer");
+    BB_HIP(s->ctx, hipGraphLaunch(s->gexec[s->cur], s->ctx->stream));
+    s->cur ^= s->g_flip;
+    return BB_OK;
+}
+
+int bb_solver_set_tuning(bb_solver *s, int prec, int elems_per_block) {
+    if (!s) return BB_ERR_NULL;
+    if (prec != BB_PREC_F64 && prec != BB_PREC_MIXED) return bb_fail(s->ctx, BB_ERR_ARG, "prec %d", prec);
+    bool ok = false;
+    for (int e : kStageEB) ok |= e == elems_per_block;
+    if (!ok) return bb_fail(s->ctx, BB_ERR_ARG, "elements per block %d not compiled", elems_per_block);
+    s->eb[prec] = elems_per_block;
+    drop_graphs(s);
+    return BB_OK;
+}
+
+int bb_solver_autotune(bb_solver *s, int prec, int *elems_per_block) {
+    if (!s) return BB_ERR_NULL;
+    int rc = check_path(s, BB_PATH_FUSED, prec);
+    if (rc) return rc;
+    bb_ctx *ctx = s->ctx;
+    StageArgs a = stage_args(s, prec);
+    a.q_in = s->q[s->cur];
+    a.q_out = s->q[s->cur ^ 1];  // the current state is only read
+    a.ca = kA[1];
+    a.cb = kB[1];
+    a.dt = 0.0;
+    a.first = 0;
+    float best = 1e30f;
+    int best_eb = 0;
+    for (int e : kStageEB) {
+        if (e * s->NP > 1024) break;
+        if (stage_fn(s->pde, prec)(s->N, e, a, ctx->stream) != BB_OK) continue;  // not compiled: skip
+        float ms = 0.f;
+        BB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        BB_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        for (int r = 0; r < 5; ++r) stage_fn(s->pde, prec)(s->N, e, a, ctx->stream);
+        BB_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        BB_HIP(ctx, hipEventSynchronize(ctx->ev1));
+        BB_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (ms < best) {
+            best = ms;
+            best_eb = e;
+        }
+    }
+    if (!best_eb) return bb_fail(ctx, BB_ERR_UNSUPPORTED, "no fused kernel compiled for N=%d", s->N);
+    s->eb[prec] = best_eb;
+    drop_graphs(s);
+    if (elems_per_block) *elems_per_block = best_eb;
+    return BB_OK;
+}
+
+int bb_solver_stage_bytes(bb_solver *s, int prec, double *bytes, double *dofs) {
+    if (!s) return BB_ERR_NULL;
+    const double w = (double)state_words(s);
+    const double sres = prec == BB_PREC_MIXED ? 4.0 : 8.0;
+    // q in + q out (FP64), residual in + out, per-element geometry / normals / connectivity
+    if (bytes) *bytes = w * (8.0 + 8.0 + 2.0 * sres) + (double)s->K * (9 * 8 + 8 + 12 * 8 + 4 * 8);
+    if (dofs) *dofs = w;
+    return BB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,5 @@
+// bb_stage_wave_f64.hip -- instantiations of the fused stage kernel (BB_PDE_WAVE, mixed=false).
+#include "bb_dispatch.hpp"
+#include "bb_stage.hpp"
+
+BB_STAGE_DEFINE(stage_launch_wave_f64, BB_PDE_WAVE, false)

@@ -1,0 +1,107 @@
+// bb_stencil.hpp -- the barycentric derivative stencils and face-lift building blocks.
+#pragma once
+#include <type_traits>
+
+#include "bb_common.hpp"
+
+namespace bb {
+
+// linear index -> (i,j,k) for degree N (k outer, j, i inner)
+template <int N> __device__ __forceinline__ void decode3(int n, int &i, int &j, int &k) {
+    int kk = 0;
+    while (n >= nfp2(N - kk)) { n -= nfp2(N - kk); ++kk; }
+    int jj = 0;
+    while (n >= N - kk - jj + 1) { n -= N - kk - jj + 1; ++jj; }
+    i = n; j = jj; k = kk;
+}
+// linear index -> (a,b) for a degree-n triangle (b outer, a inner)
+__host__ __device__ __forceinline__ void decode2(int n, int m, int &a, int &b) {
+    int bb_ = 0;
+    while (m >= n - bb_ + 1) { m -= n - bb_ + 1; ++bb_; }
+    a = m; b = bb_;
+}
+
+// All three reference-coordinate derivatives of one coefficient vector x (degree N,
+// linear order) at node idx = (i,j,k,l):
+//   Dr = 1/2 [ i(x_a - x_{a-e0+e3}) + j(x_{a-e1+e0} - x_{a-e1+e3}) + k(...) + l(x_{a-e3+e0} - x_a) ]
+// exactly the four weighted differences of fast_Dr_multiply! (src/BernsteinDerivativeMatrix_3D.jl:96-132),
+// fast_Ds_multiply! (:138-174) and fast_Dt_multiply! (:180-216), accumulated in the same
+// order.  A term whose weight is zero reads the centre value instead of an index that
+// does not exist (branch-free), and contributes 0 * finite = 0.
+template <int N, typename T>
+__device__ __forceinline__ void grad_ref(const T *x, int idx, int i, int j, int k, T &dr, T &ds, T &dt) {
+    const int l = N - i - j - k;
+    const int rl = N - k - j + 1;                    // length of row j in slab k
+    const int sk = (N - k + 1) * (N - k + 2) / 2;    // size of slab k
+    const int skm = (N - k + 2) * (N - k + 3) / 2;   // size of slab k-1
+    const T xc = x[idx];
+    const T xim = x[i > 0 ? idx - 1 : idx];          // (i-1,j,k)
+    const T xip = x[l > 0 ? idx + 1 : idx];          // (i+1,j,k)
+    const T xjm = x[j > 0 ? idx - rl - 1 : idx];     // (i,j-1,k)
+    const T xjmip = x[j > 0 ? idx - rl : idx];       // (i+1,j-1,k)
+    const T xjp = x[l > 0 ? idx + rl : idx];         // (i,j+1,k)
+    const T xjpim = x[i > 0 ? idx + rl - 1 : idx];   // (i-1,j+1,k)
+    const int km0 = idx - skm + j;                   // (i,j,k-1)
+    const T xkm = x[k > 0 ? km0 : idx];
+    const T xkmip = x[k > 0 ? km0 + 1 : idx];        // (i+1,j,k-1)
+    const T xkmjp = x[k > 0 ? km0 + rl + 1 : idx];   // (i,j+1,k-1)
+    const int kp0 = idx + sk - j;                    // (i,j,k+1)
+    const T xkp = x[l > 0 ? kp0 : idx];
+    const T xkpim = x[i > 0 ? kp0 - 1 : idx];        // (i-1,j,k+1)
+    const T xkpjm = x[j > 0 ? kp0 - rl : idx];       // (i,j-1,k+1)
+    const T ti = T(i), tj = T(j), tk = T(k), tl = T(l), h = T(0.5);
+    T v = T(0);
+    v += ti * (xc - xim);
+    v += tj * (xjmip - xjm);
+    v += tk * (xkmip - xkm);
+    v += tl * (xip - xc);
+    dr = h * v;
+    v = T(0);
+    v += ti * (xjpim - xim);
+    v += tj * (xc - xjm);
+    v += tk * (xkmjp - xkm);
+    v += tl * (xjp - xc);
+    ds = h * v;
+    v = T(0);
+    v += ti * (xkpim - xim);
+    v += tj * (xkpjm - xjm);
+    v += tk * (xc - xkm);
+    v += tl * (xkp - xc);
+    dt = h * v;
+}
+
+// L0 x at face node (a,b) of degree N:  L0 = (N+1)^2/2 (E_N^{N+1})^T E_N^{N+1}
+// (src/BernsteinLift.jl:141-145), which is the 7-point stencil
+//   1/2 [ ((a+1)^2+(b+1)^2+(c+1)^2) x_{ab} + (a+1) b x_{a+1,b-1} + (a+1) c x_{a+1,b}
+//         + (b+1) a x_{a-1,b+1} + (b+1) c x_{a,b+1} + (c+1) a x_{a-1,b} + (c+1) b x_{a,b-1} ]
+// (degree-N face layout; a weight is zero exactly when its neighbour does not exist).
+template <int N, typename T> __device__ __forceinline__ T l0_apply(const T *x, int a, int b) {
+    const int c = N - a - b;
+    const int idx = lin2(N, a, b);
+    const int rl = N - b + 1;  // row b length
+    const T x0 = x[idx];
+    const T xap_bm = x[b > 0 ? idx - rl : idx];       // (a+1,b-1)
+    const T xap = x[c > 0 ? idx + 1 : idx];           // (a+1,b)
+    const T xam_bp = x[a > 0 ? idx + rl - 1 : idx];   // (a-1,b+1)
+    const T xbp = x[c > 0 ? idx + rl : idx];          // (a,b+1)
+    const T xam = x[a > 0 ? idx - 1 : idx];           // (a-1,b)
+    const T xbm = x[b > 0 ? idx - rl - 1 : idx];      // (a,b-1)
+    T s = T((a + 1) * (a + 1) + (b + 1) * (b + 1) + (c + 1) * (c + 1)) * x0;
+    s += T((a + 1) * b) * xap_bm;
+    s += T((a + 1) * c) * xap;
+    s += T((b + 1) * a) * xam_bp;
+    s += T((b + 1) * c) * xbp;
+    s += T((c + 1) * a) * xam;
+    s += T((c + 1) * b) * xbm;
+    return T(0.5) * s;
+}
+
+// (E_{d}^{d+1})^T x: degree d+1 -> degree d at (a,b), reduction_multiply!
+// (src/BernsteinLift.jl:66-90): ((a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (d+1-a-b) x_{ab}) / (d+1)
+template <typename T> __device__ __forceinline__ T reduce_apply(const T *x, int d, int a, int b, T inv) {
+    const int idx = lin2(d + 1, a, b);
+    const T v = fma(T(a + 1), x[idx + 1], fma(T(b + 1), x[idx + (d + 2 - b)], T(d + 1 - a - b) * x[idx]));
+    return v * inv;
+}
+
+}  // namespace bb

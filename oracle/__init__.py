@@ -72,6 +72,7 @@ def lib():
         L.orc_lsrk_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, dp, C.c_double, i32p, i8p, dp, dp, dp, dp,
                                     dp, dp, C.c_int]
         L.orc_lsrk_coeffs.argtypes = [dp, dp]
+        L.orc_flux.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, i32p, i8p, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -197,6 +198,24 @@ def rhs(pde, N, q_aos, ma: MeshArrays, nthreads=1, flux_kind=1):
                   ma.nx.reshape(-1), ma.ny.reshape(-1), ma.nz.reshape(-1), ma.geo.reshape(-1), ma.J, du.reshape(-1),
                   nthreads)
     return du
+
+
+def face_gather(N, q_aos):
+    """uM = q[Fmask, :] on (K, Np, F) -> (K, 4Nfp, F)"""
+    q = np.ascontiguousarray(q_aos, dtype=np.float64)
+    K, _, F = q.shape
+    uM = np.zeros((K, 4 * Nfp(N), F))
+    lib().orc_face_gather(N, F, K, q.reshape(-1), uM.reshape(-1))
+    return uM
+
+
+def flux(pde, N, uM, ma: MeshArrays, flux_kind=1):
+    """interface flux on (K, 4Nfp, F)"""
+    uM = np.ascontiguousarray(uM, dtype=np.float64)
+    out = np.zeros_like(uM)
+    lib().orc_flux(pde, flux_kind, N, uM.shape[0], uM.reshape(-1), ma.mapP.reshape(-1), ma.bc.reshape(-1),
+                   ma.bc_sign, ma.nx.reshape(-1), ma.ny.reshape(-1), ma.nz.reshape(-1), out.reshape(-1))
+    return out
 
 
 def lsrk_step(pde, N, q_aos, dt, ma: MeshArrays, nthreads=1, flux_kind=1):

@@ -606,3 +606,12 @@ void orc_lsrk_step(int pde, int flux_kind, int N, int64_t K, double *q, double *
     }
     free(du);
 }
+
+/* the interface-flux loop alone (examples/3d_wave.jl:43-56 / 3d_maxwell.jl:187-209),
+ * uM and flux in [K][4Nfp][F] */
+void orc_flux(int pde, int flux_kind, int N, int64_t K, const double *uM, const int32_t *mapP, const int8_t *bc,
+              const double *bc_sign, const double *nxJ, const double *nyJ, const double *nzJ, double *flux) {
+    int F = pde == 0 ? 4 : 6;
+    orc_mesh_t M = {N, F, pde, flux_kind, K, mapP, bc, nxJ, nyJ, nzJ, NULL, NULL, bc_sign};
+    orc_interface_flux(&M, uM, flux);
+}

@@ -1,0 +1,227 @@
+"""GPU parity of the operator kernels against the oracle (FP64: <= 1e-12 relative,
+FP32: <= 2e-6 relative in the norm of each output), through the C ABI."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def np3(N):
+    return (N + 1) * (N + 2) * (N + 3) // 6
+
+
+def nfp(N):
+    return (N + 1) * (N + 2) // 2
+
+
+def to_dev_cols(x_kfn, layout, dtype):
+    """x (K, F, n) host -> Julia-shaped (n, K, F) device view in the requested layout."""
+    import bbdg
+
+    K, F, n = x_kfn.shape
+    if layout == "aos":  # Julia column-major SVector{F}: [K][n][F]
+        d = bbdg.DeviceArray.from_numpy(np.ascontiguousarray(x_kfn.transpose(0, 2, 1), dtype=dtype))
+        return d, d.view((n, K, F), (F, n * F, 1))
+    d = bbdg.DeviceArray.from_numpy(np.ascontiguousarray(x_kfn, dtype=dtype))  # [K][F][n]
+    return d, d.view((n, K, F), (1, F * n, n))
+
+
+def from_dev_cols(v):
+    """Julia-shaped (n, K, F) device view -> (K, F, n) host"""
+    return v.numpy().transpose(1, 2, 0)
+
+
+def empty_cols(n, K, F, layout, dtype):
+    import bbdg
+
+    if layout == "aos":
+        d = bbdg.DeviceArray.zeros((K, n, F), dtype)
+        return d, d.view((n, K, F), (F, n * F, 1))
+    d = bbdg.DeviceArray.zeros((K, F, n), dtype)
+    return d, d.view((n, K, F), (1, F * n, n))
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 7, 9, 12, 20])
+@pytest.mark.parametrize("layout", ["aos", "soa"])
+def test_deriv_fp64(ctx, orc, N, layout):
+    import bbdg
+
+    rng = np.random.default_rng(100 + N)
+    K, F = 7, 3
+    x = rng.uniform(-1, 1, (K, F, np3(N)))
+    _, xv = to_dev_cols(x, layout, np.float64)
+    for axis, D in enumerate([bbdg.BernsteinDerivativeMatrix_3D_r(N), bbdg.BernsteinDerivativeMatrix_3D_s(N),
+                              bbdg.BernsteinDerivativeMatrix_3D_t(N)]):
+        _, ov = empty_cols(np3(N), K, F, layout, np.float64)
+        bbdg.mul_(ov, D, xv)
+        got = from_dev_cols(ov)
+        ref = orc.deriv_mul(axis, N, x.reshape(-1, np3(N))).reshape(K, F, -1)
+        assert rel(got, ref) <= TOL64, (N, axis, layout, rel(got, ref))
+
+
+@pytest.mark.parametrize("N", [1, 4, 7, 9])
+def test_deriv_vector_and_fp32(ctx, orc, N):
+    import bbdg
+
+    rng = np.random.default_rng(7)
+    n = np3(N)
+    x = rng.uniform(-1, 1, n)
+    D = bbdg.BernsteinDerivativeMatrix_3D_t(N)
+    xd = bbdg.DeviceArray.from_numpy(x)
+    od = bbdg.DeviceArray.zeros((n,))
+    bbdg.mul_(od, D, xd)
+    assert rel(od.numpy(), orc.deriv_mul(2, N, x[None])[0]) <= TOL64
+    x32 = bbdg.DeviceArray.from_numpy(x.astype(np.float32))
+    o32 = bbdg.DeviceArray.zeros((n,), np.float32)
+    bbdg.mul_(o32, D, x32)
+    assert rel(o32.numpy(), orc.deriv_mul(2, N, x.astype(np.float32).astype(np.float64)[None])[0]) <= 2e-6
+
+
+@pytest.mark.parametrize("N", [1, 3, 7, 9])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, TOL64), (np.float32, 2e-6)])
+def test_grad(ctx, orc, N, dtype, tol):
+    import bbdg
+    from bbdg import mesh
+
+    m = mesh.kuhn_mesh(2, 0.1, 1)
+    K, F = m.K, 4
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, (K, F, np3(N)))
+    _, xv = to_dev_cols(x, "soa", dtype)
+    outs = [empty_cols(np3(N), K, F, "soa", dtype) for _ in range(3)]
+    geo = bbdg.DeviceArray.from_numpy(m.geo)
+    bbdg.grad_(outs[0][1], outs[1][1], outs[2][1], N, xv, geo)
+    xr = x.astype(dtype).astype(np.float64).reshape(-1, np3(N))
+    d = [orc.deriv_mul(a, N, xr).reshape(K, F, -1) for a in range(3)]
+    g = m.geo[:, :, None, None]
+    for c in range(3):
+        ref = g[:, 3 * c + 0] * d[0] + g[:, 3 * c + 1] * d[1] + g[:, 3 * c + 2] * d[2]
+        assert rel(from_dev_cols(outs[c][1]), ref) <= tol, (N, c, rel(from_dev_cols(outs[c][1]), ref))
+
+
+@pytest.mark.parametrize("N", list(range(1, 13)) + [15, 17, 20])
+def test_lift_fp64(ctx, orc, N):
+    import bbdg
+
+    rng = np.random.default_rng(300 + N)
+    K, F = 5, 3
+    x = rng.uniform(-1, 1, (K, F, 4 * nfp(N)))
+    for layout in ("aos", "soa"):
+        _, xv = to_dev_cols(x, layout, np.float64)
+        _, ov = empty_cols(np3(N), K, F, layout, np.float64)
+        bbdg.mul_(ov, bbdg.BernsteinLift(N), xv)
+        ref = orc.lift(N, x.reshape(-1, 4 * nfp(N))).reshape(K, F, -1)
+        assert rel(from_dev_cols(ov), ref) <= TOL64, (N, layout, rel(from_dev_cols(ov), ref))
+
+
+@pytest.mark.parametrize("N", [2, 5, 7, 9])
+@pytest.mark.parametrize("mode", ["plain", "acc64"])
+def test_lift_fp32(ctx, orc, N, mode):
+    import bbdg
+
+    rng = np.random.default_rng(N)
+    K, F = 9, 2
+    x = rng.uniform(-1, 1, (K, F, 4 * nfp(N)))
+    _, xv = to_dev_cols(x, "soa", np.float32)
+    _, ov = empty_cols(np3(N), K, F, "soa", np.float32)
+    bbdg.mul_(ov, bbdg.BernsteinLift(N, np.float32, mode), xv)
+    ref = orc.lift(N, x.astype(np.float32).astype(np.float64).reshape(-1, 4 * nfp(N))).reshape(K, F, -1)
+    assert rel(from_dev_cols(ov), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("N", [1, 2, 5, 8, 11, 14, 17, 20])
+def test_face_lift(ctx, orc, N):
+    import bbdg
+
+    rng = np.random.default_rng(400 + N)
+    K = 6
+    x = rng.standard_normal((K, 1, nfp(N)))
+    ref = orc.face_lift(N, x.reshape(-1, nfp(N))).reshape(K, 1, -1)
+    L = bbdg.BernsteinLift(N)
+    _, xv = to_dev_cols(x, "soa", np.float64)
+    _, ov = empty_cols(np3(N), K, 1, "soa", np.float64)
+    bbdg.face_lift_multiply_(ov, L, xv)
+    assert rel(from_dev_cols(ov), ref) <= TOL64
+    # FP32 input, FP64 output: the chain precision variants
+    ref32 = orc.face_lift(N, x.astype(np.float32).astype(np.float64).reshape(-1, nfp(N))).reshape(K, 1, -1)
+    _, x32 = to_dev_cols(x, "soa", np.float32)
+    errs = {}
+    for mode in ("plain", "acc64", "twosum"):
+        _, ov = empty_cols(np3(N), K, 1, "soa", np.float64)
+        bbdg.face_lift_multiply_(ov, L, x32, mode=mode)
+        errs[mode] = rel(from_dev_cols(ov), ref32)
+    assert errs["plain"] <= 1e-5
+    assert errs["acc64"] <= 1e-5 and errs["twosum"] <= 1e-5
+
+
+@pytest.mark.parametrize("N", [1, 2, 4, 7, 10, 20])
+def test_elevation(ctx, orc, N):
+    import bbdg
+
+    rng = np.random.default_rng(N)
+    K, F = 4, 2
+    x = rng.uniform(-1, 1, (K, F, nfp(N - 1)))
+    _, xv = to_dev_cols(x, "aos", np.float64)
+    _, ov = empty_cols(nfp(N), K, F, "aos", np.float64)
+    bbdg.mul_(ov, bbdg.ElevationMatrix(N), xv)
+    ref = np.stack([[orc.elevation_multiply(N - 1, x[k, f]) for f in range(F)] for k in range(K)])
+    assert rel(from_dev_cols(ov), ref) <= TOL64
+
+
+def test_face_gather(ctx, orc):
+    import bbdg
+
+    N, K, F = 6, 5, 4
+    rng = np.random.default_rng(0)
+    q = rng.uniform(-1, 1, (K, F, np3(N)))
+    _, qv = to_dev_cols(q, "aos", np.float64)
+    _, uv = empty_cols(4 * nfp(N), K, F, "aos", np.float64)
+    bbdg.face_gather_(uv, N, qv)
+    fm = orc.fmask(N).reshape(-1)
+    assert np.array_equal(from_dev_cols(uv), q[:, :, fm])
+
+
+@pytest.mark.parametrize("pde", ["wave", "maxwell"])
+@pytest.mark.parametrize("flux_kind", ["reference", "upwind"])
+def test_interface_flux(ctx, orc, pde, flux_kind):
+    import bbdg
+    from bbdg import mesh
+
+    N = 4
+    F = 4 if pde == "wave" else 6
+    periodic = pde == "wave"
+    m = mesh.kuhn_mesh(2, 0.1, 3, periodic=periodic)
+    sign = [1.0] * F if pde == "wave" else [1, 1, 1, -1, -1, -1]
+    ma = orc.MeshArrays(m, N, sign)
+    rng = np.random.default_rng(5)
+    q = rng.uniform(-1, 1, (m.K, np3(N), F))
+    uM = orc.face_gather(N, q)
+    d_uM = bbdg.DeviceArray.from_numpy(uM)
+    uMv = d_uM.view((4 * nfp(N), m.K, F), (F, 4 * nfp(N) * F, 1))
+    d_fl = bbdg.DeviceArray.zeros(uM.shape)
+    flv = d_fl.view(uMv.shape, uMv.strides)
+    dev = {k: bbdg.DeviceArray.from_numpy(v) for k, v in
+           dict(mapP=ma.mapP, nx=ma.nx, ny=ma.ny, nz=ma.nz, bc=ma.bc).items()}
+    bbdg.interface_flux_(flv, pde, N, uMv, dev["mapP"], dev["nx"], dev["ny"], dev["nz"], bc=dev["bc"], bc_sign=sign,
+                         flux_kind=flux_kind)
+    ref = orc.flux(0 if pde == "wave" else 1, N, uM, ma, flux_kind=0 if flux_kind == "reference" else 1)
+    assert rel(d_fl.numpy(), ref) <= TOL64
+
+
+def test_errors_are_reported(ctx):
+    import bbdg
+
+    D = bbdg.BernsteinDerivativeMatrix_3D_r(3)
+    x = bbdg.DeviceArray.zeros((20, 2))
+    o = bbdg.DeviceArray.zeros((21, 2))
+    with pytest.raises(ValueError):
+        bbdg.mul_(o, D, x)
+    lay = bbdg._lib.bb_layout(20, 1, 0)
+    rc = bbdg.lib().bb_deriv_apply(ctx.h, 0, 21, 1, 2, 0, x.ptr, lay, x.ptr, lay)
+    assert rc == bbdg._lib.BB_ERR_ARG and b"outside" in bbdg.lib().bb_last_error(ctx.h)

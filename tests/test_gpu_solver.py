@@ -1,0 +1,123 @@
+"""GPU parity of the solver paths (plain op-by-op FP64, fused FP64, fused mixed) against
+the oracle's restatement of examples/3d_wave.jl / 3d_maxwell.jl + LSRK45."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def setup(pde, N, n=2, seed=0, periodic=None, flux="upwind"):
+    import bbdg
+    import oracle as O
+    from bbdg import basis, mesh
+
+    if periodic is None:
+        periodic = pde == "wave"
+    m = mesh.kuhn_mesh(n, 0.1, seed, periodic=periodic)
+    if pde == "wave":
+        fn = lambda x, y, z: np.stack([np.exp(-25 * ((x - .5) ** 2 + (y - .5) ** 2 + (z - .5) ** 2)),
+                                       0.3 * np.sin(2 * np.pi * y), 0 * x, 0.1 * np.cos(2 * np.pi * x)])
+        sign = [1.0] * 4
+    else:
+        fn = lambda x, y, z: np.stack([0.2 * np.sin(np.pi * z), 0 * x, 0.1 * x * y, 0 * x, 0.05 * z,
+                                       np.sin(np.pi * x) * np.sin(np.pi * y)])
+        sign = [1, 1, 1, -1, -1, -1]
+    q0 = np.ascontiguousarray(basis.interpolate(m, N, fn))
+    s = bbdg.Solver(m, N, pde, flux=flux)
+    s.set_state(q0)
+    ma = O.MeshArrays(m, N, sign)
+    return m, q0, s, ma, O
+
+
+@pytest.mark.parametrize("pde,N", [("wave", 1), ("wave", 3), ("wave", 5), ("wave", 7), ("wave", 9), ("wave", 10),
+                                   ("maxwell", 2), ("maxwell", 5), ("maxwell", 9)])
+@pytest.mark.parametrize("flux", ["upwind", "reference"])
+def test_rhs_parity(ctx, pde, N, flux):
+    m, q0, s, ma, O = setup(pde, N, flux=flux)
+    p = 0 if pde == "wave" else 1
+    ref = O.rhs(p, N, q0, ma, nthreads=4, flux_kind=1 if flux == "upwind" else 0)
+    for path in ("plain", "fused"):
+        got = s.rhs(path, "f64")
+        assert rel(got, ref) <= TOL64, (path, rel(got, ref))
+    mixed = s.rhs("fused", "mixed")
+    assert rel(mixed, ref) <= 2e-5, rel(mixed, ref)
+
+
+@pytest.mark.parametrize("pde,N", [("wave", 5), ("wave", 7), ("maxwell", 9)])
+def test_step_parity(ctx, pde, N):
+    m, q0, s, ma, O = setup(pde, N)
+    p = 0 if pde == "wave" else 1
+    dt = 0.5 * m.h / N ** 2
+    qr = q0.copy()
+    for _ in range(3):
+        O.lsrk_step(p, N, qr, dt, ma, nthreads=4, flux_kind=1)
+    for path, prec, tol in (("plain", "f64", TOL64), ("fused", "f64", TOL64), ("fused", "mixed", 1e-5)):
+        s.set_state(q0)
+        s.step(dt, 3, path, prec)
+        got = s.get_state()
+        assert rel(got, qr) <= tol, (path, prec, rel(got, qr))
+
+
+def test_graph_replay_matches_eager(ctx):
+    m, q0, s, ma, O = setup("wave", 7, n=3)
+    dt = 0.5 * m.h / 49
+    s.step(dt, 6, "fused", "f64")
+    eager = s.get_state()
+    for nsteps in (3, 2):  # odd and even stage counts per graph
+        s.set_state(q0)
+        s.capture(dt, nsteps, "fused", "f64")
+        for _ in range(6 // nsteps):
+            s.launch()
+            s.get_state()  # eager host work between replays
+        assert np.array_equal(s.get_state(), eager)
+    s.set_state(q0)
+    s.capture(dt, 2, "plain", "f64")
+    for _ in range(3):
+        s.launch()
+    assert rel(s.get_state(), eager) <= TOL64
+
+
+def test_autotune_and_tiling_invariance(ctx):
+    m, q0, s, ma, O = setup("wave", 5, n=3)
+    outs = []
+    for eb in (1, 2, 3, 4):
+        s.set_tuning("f64", eb)
+        outs.append(s.rhs("fused", "f64"))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    eb = s.autotune("f64")
+    assert eb in (1, 2, 3, 4, 6, 8)
+    assert np.array_equal(s.rhs("fused", "f64"), outs[0])
+
+
+def test_config1_every_step(ctx):
+    """3-D acoustic wave, N=5, 8^3 x 6 jittered Kuhn mesh (seed 0), periodic,
+    p = exp(-25|x-0.5|^2), LSRK45 dt = 0.5 h / N^2, 50 steps; fields compared after every
+    step against the oracle (FP64 fused path)."""
+    import bbdg
+    import oracle as O
+    from bbdg import basis, mesh
+
+    N = 5
+    m = mesh.kuhn_mesh(8, 0.1, 0, periodic=True)
+    q0 = np.ascontiguousarray(basis.interpolate(m, N, lambda x, y, z: np.stack(
+        [np.exp(-25 * ((x - .5) ** 2 + (y - .5) ** 2 + (z - .5) ** 2)), 0 * x, 0 * x, 0 * x])))
+    s = bbdg.Solver(m, N, "wave")
+    s.set_state(q0)
+    ma = O.MeshArrays(m, N, [1, 1, 1, 1])
+    dt = 0.5 * m.h / N ** 2
+    qr = q0.copy()
+    worst = 0.0
+    for step in range(50):
+        O.lsrk_step(0, N, qr, dt, ma, nthreads=8, flux_kind=1)
+        s.step(dt, 1, "fused", "f64")
+        e = rel(s.get_state(), qr)
+        worst = max(worst, e)
+        assert e <= TOL64, (step, e)
+    print(f"config 1: worst per-step relative L2 difference {worst:.3e}")
