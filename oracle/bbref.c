@@ -62,6 +62,21 @@ void orc_tet_offsets(int N, int64_t out[21]) {
 /* src/ordering.jl:6  ij_to_linear(i,j,tri) = i + tri[j+1] + 1  (1-based result) */
 int64_t orc_ij_to_linear(int i, int j, const int64_t tri[21]) { return i + tri[j] + 1; }
 
+/* The same tables continued to 24 entries for internal use: reduction_multiply! of degree
+ * N+1 (BernsteinLift.jl:142) reads row N+1 of tri_offsets(N+1), which for N = 20 lies past
+ * the reference's 21-entry tuple (the reference raises a BoundsError there; the oracle
+ * computes the mathematically defined value). */
+static void tri_ext(int N, int64_t out[24]) {
+    int64_t count = 0;
+    out[0] = 0;
+    for (int i = 1; i < 24; ++i) { count += N + 2 - i; out[i] = count; }
+}
+static void tet_ext(int N, int64_t out[24]) {
+    int64_t count = 0;
+    out[0] = 0;
+    for (int i = 1; i < 24; ++i) { count += ((int64_t)(N + 2 - i) * (N + 3 - i)) / 2; out[i] = count; }
+}
+
 /* src/ordering.jl:33-35  ijk_to_linear = i + tri[j+1] + 1 + tet[k+1] - j*k  (1-based result) */
 int64_t orc_ijk_to_linear(int i, int j, int k, const int64_t tri[21], const int64_t tet[21]) {
     return i + tri[j] + 1 + tet[k] - (int64_t)j * k;
@@ -154,9 +169,9 @@ double orc_deriv_entry(int axis, int N, int m, int n) {
  * :180-216 fast_Dt_multiply!  (one column; x, out of length Np, strided by `stride`
  * so that an SVector{F} column can be processed field by field). */
 void orc_deriv_mul_strided(int axis, int N, const double *x, double *out, int stride) {
-    int64_t tri[21], tet[21];
-    orc_tri_offsets(N, tri);
-    orc_tet_offsets(N, tet);
+    int64_t tri[24], tet[24];
+    tri_ext(N, tri);
+    tet_ext(N, tet);
     int64_t row = 0;
     for (int k = 0; k <= N; ++k)
         for (int j = 0; j <= N - k; ++j)
@@ -260,8 +275,8 @@ double orc_elev_entry(int N, int m, int n) {
 /* src/BernsteinLift.jl:100-125  elevation_multiply!(out, N, x, tri_offsets(N)):
  * degree N (Nfp(N)) -> degree N+1 (Nfp(N+1)) */
 void orc_elevation_multiply(int N, const double *x, double *out) {
-    int64_t tri[21];
-    orc_tri_offsets(N, tri);
+    int64_t tri[24];
+    tri_ext(N, tri);
     int row = 0;
     for (int j = 0; j <= N + 1; ++j)
         for (int i = 0; i <= N + 1 - j; ++i) {
@@ -277,8 +292,8 @@ void orc_elevation_multiply(int N, const double *x, double *out) {
 /* src/BernsteinLift.jl:66-90  reduction_multiply!(out, N, x, tri_offsets(N)):
  * degree N -> degree N-1.  Safe in place (out == x), as the reference uses it. */
 void orc_reduction_multiply(int N, const double *x, double *out) {
-    int64_t tri[21];
-    orc_tri_offsets(N, tri);
+    int64_t tri[24];
+    tri_ext(N, tri);
     int row = 0;
     for (int j = 0; j <= N - 1; ++j)
         for (int i = 0; i <= N - 1 - j; ++i) {
@@ -315,9 +330,9 @@ void orc_face_lift(int N, const double *x, double *out, double *E, double *Fs) {
 /* src/BernsteinLift.jl:191-220  mul!(out, L::BernsteinLift, x): x = [f1; f2; f3; f4],
  * each Nfp(N); out Np.  scratch: Nfp(N) + Nfp(N+1) + 3*Np doubles. */
 void orc_lift(int N, const double *x, double *out, double *scratch) {
-    int64_t tri[21], tet[21];
-    orc_tri_offsets(N, tri);
-    orc_tet_offsets(N, tet);
+    int64_t tri[24], tet[24];
+    tri_ext(N, tri);
+    tet_ext(N, tet);
     int nfp = orc_nfp(N), np = orc_np(N);
     double *E = scratch, *Fs = E + nfp, *G1 = Fs + orc_nfp(N + 1), *G2 = G1 + np, *G3 = G2 + np;
     orc_face_lift(N, x + 3 * nfp, out, E, Fs);
@@ -367,9 +382,9 @@ void orc_face_lift_cols(int N, int64_t ncols, const double *x, double *out) {
  * This plays the role of rd.Fmask (examples/3d_wave.jl:41): the trace of a Bernstein
  * polynomial on a face is the sub-vector of coefficients with zero exponent there. */
 int orc_face_node_volume_index(int N, int f, int a, int b) {
-    int64_t tri[21], tet[21];
-    orc_tri_offsets(N, tri);
-    orc_tet_offsets(N, tet);
+    int64_t tri[24], tet[24];
+    tri_ext(N, tri);
+    tet_ext(N, tet);
     int c = N - a - b;
     switch (f) {
     case 0: return (int)LIN(a, 0, b);
