@@ -88,10 +88,10 @@ def interpolate(mesh, N, fn):
     Vinv, _ = interpolation_matrix(N)
     X = domain_points(mesh, N)  # (K,Np,3)
     vals = np.asarray(fn(X[..., 0], X[..., 1], X[..., 2]), dtype=np.float64)  # (F,K,Np)
-    return np.einsum("ab,fkb->kaf", Vinv, vals)
+    return np.ascontiguousarray(np.matmul(vals, Vinv.T).transpose(1, 2, 0))
 
 
 def evaluate(N, coeffs):
     """Point values at the equispaced domain points: (K,Np,F) -> (K,Np,F)."""
     _, V = interpolation_matrix(N)
-    return np.einsum("ab,kbf->kaf", V, coeffs)
+    return np.matmul(V, coeffs)

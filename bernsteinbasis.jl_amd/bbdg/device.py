@@ -64,6 +64,20 @@ def default_context() -> Context:
 
 
 _DT = {np.dtype(np.float64): B.BB_F64, np.dtype(np.float32): B.BB_F32}
+# dtypes a device array may hold (floating point data and the integer index tables)
+STORABLE = {np.dtype(t) for t in (np.float64, np.float32, np.int64, np.int32, np.int16, np.int8, np.uint8)}
+
+
+def host_buffer(a):
+    """(contiguous 1-D buffer in memory order, dtype, strides in elements) for an upload.
+    The dtype is kept as given; only non-storable dtypes are refused."""
+    a = np.asarray(a)
+    if a.dtype not in STORABLE:
+        raise TypeError(f"cannot store dtype {a.dtype} on the device")
+    if not (a.flags.c_contiguous or a.flags.f_contiguous):
+        a = np.ascontiguousarray(a)
+    strides = tuple(s // a.itemsize for s in a.strides)
+    return np.ascontiguousarray(a.ravel(order="K")), a.dtype, strides, a.shape
 
 
 class DeviceArray:
@@ -103,6 +117,8 @@ class DeviceArray:
 
     @property
     def code(self):
+        if self.dtype not in _DT:
+            raise TypeError(f"operators take float64 or float32 data, not {self.dtype}")
         return _DT[self.dtype]
 
     def free(self):
@@ -118,14 +134,8 @@ class DeviceArray:
 
     @classmethod
     def from_numpy(cls, a, ctx: Context | None = None):
-        a = np.asarray(a)
-        if a.dtype not in _DT:
-            a = a.astype(np.float64)
-        if not (a.flags.c_contiguous or a.flags.f_contiguous):
-            a = np.ascontiguousarray(a)
-        strides = tuple(s // a.itemsize for s in a.strides)
-        d = cls(a.shape, a.dtype, strides, ctx)
-        buf = np.ascontiguousarray(a.ravel(order="K"))
+        buf, dtype, strides, shape = host_buffer(a)
+        d = cls(shape, dtype, strides, ctx)
         d.ctx.check(B.lib().bb_memcpy_h2d(d.ctx.h, C.c_void_p(d.ptr), buf.ctypes.data_as(C.c_void_p), buf.nbytes))
         return d
 

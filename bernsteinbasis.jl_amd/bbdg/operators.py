@@ -305,6 +305,13 @@ def interface_flux_(flux: DeviceArray, pde: str, N: int, uM: DeviceArray, mapP: 
     lf, kf, Ff = _cols(flux, 4 * _nfp(N), "flux")
     if Fm != F or (km, Fm) != (kf, Ff):
         raise ValueError("uM/flux shape")
+    if uM.dtype != np.float64 or flux.dtype != np.float64:
+        raise TypeError("the interface flux is FP64")
+    M = 4 * _nfp(N)
+    for name, arr, dt in (("mapP", mapP, np.int32), ("nx", nx, np.float64), ("ny", ny, np.float64),
+                          ("nz", nz, np.float64)) + ((("bc", bc, np.int8),) if bc is not None else ()):
+        if arr.dtype != np.dtype(dt) or arr.shape != (km, M) or arr.strides != (M, 1):
+            raise TypeError(f"{name} must be a C-contiguous ({km}, {M}) {np.dtype(dt).name} array")
     sign = None
     if bc is not None:
         sign = np.ascontiguousarray(np.asarray(bc_sign if bc_sign is not None else np.ones(F), np.float64))

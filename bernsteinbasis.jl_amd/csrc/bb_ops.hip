@@ -156,7 +156,8 @@ __global__ void k_interface_flux(int kind, int N, const double *__restrict__ uM,
 #pragma unroll
             for (int f = 0; f < F; ++f) d[f] = sg.s[f] * um[f] - um[f];
         } else {
-            const int64_t p = mapP[idx];
+            int64_t p = mapP[idx];
+            if (p < 0 || p >= total) p = idx;  // an invalid index reads the own trace (zero jump)
             const double *pp = uM + (p / M4) * lm.e + (int64_t)(p % M4) * lm.n;
 #pragma unroll
             for (int f = 0; f < F; ++f) d[f] = pp[(int64_t)f * lm.f] - um[f];

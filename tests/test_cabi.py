@@ -77,3 +77,17 @@ def test_argument_errors_need_no_gpu():
         bbdg.BernsteinDerivativeMatrix_3D_r(2)[10, 0]
     with pytest.raises(ValueError):
         bbdg.BernsteinLift(21)
+
+
+def test_uploads_keep_integer_dtypes():
+    """index tables (mapP int32, bc int8) must reach the device with their own dtype"""
+    from bbdg.device import host_buffer
+
+    for dt in (np.int32, np.int8, np.float32, np.float64):
+        a = np.arange(12, dtype=dt).reshape(3, 4)
+        buf, dtype, strides, shape = host_buffer(a)
+        assert dtype == np.dtype(dt) and buf.dtype == np.dtype(dt) and strides == (4, 1) and shape == (3, 4)
+        buf, dtype, strides, shape = host_buffer(np.asfortranarray(a))
+        assert strides == (1, 3) and np.array_equal(buf, a.T.reshape(-1))
+    with pytest.raises(TypeError):
+        host_buffer(np.zeros(3, np.complex128))
