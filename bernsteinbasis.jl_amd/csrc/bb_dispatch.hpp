@@ -22,7 +22,7 @@ inline unsigned grid1d(int64_t work, int per_block) { return (unsigned)((work + 
 }  // namespace bb
 
 // plain-path helpers shared by the solver (bb_ops.hip)
-int bb_pde_fdir(bb_ctx *ctx, int pde, int dir, int N, int64_t K, const double *q, double *out);
+int bb_pde_fdir3(bb_ctx *ctx, int pde, int N, int64_t K, const double *q, double *fx, double *fy, double *fz);
 int bb_plain_combine(bb_ctx *ctx, int pde, int N, int64_t K, double *du, double *const d[9], const double *geo,
                      const double *J);
 int bb_lsrk_update(bb_ctx *ctx, int64_t n, double *q, double *res, const double *du, double a, double b,

@@ -171,8 +171,13 @@ __global__ void k_interface_flux(int kind, int N, const double *__restrict__ uM,
 
 // ---- plain-path helpers (solver layout [K][F][Np], FP64)
 
+// fx(q), fy(q), fz(q) at every node in one pass (the three SVector maps of
+// examples/3d_wave.jl:59-61).  The direction is a compile-time constant in each call of
+// Pde::fdir: a run-time three-way select over the directions was miscompiled for gfx950
+// (the negated entries of the third direction were dropped), so no such select is used.
 template <int PDE>
-__global__ void k_pde_fdir(int dir, int NP, const double *__restrict__ q, double *__restrict__ out, int64_t K) {
+__global__ void k_pde_fdir3(int NP, const double *__restrict__ q, double *__restrict__ fx, double *__restrict__ fy,
+                            double *__restrict__ fz, int64_t K) {
     constexpr int F = Pde<PDE>::F;
     const int64_t total = K * NP;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
@@ -181,9 +186,15 @@ __global__ void k_pde_fdir(int dir, int NP, const double *__restrict__ q, double
         double qv[F], o[F];
 #pragma unroll
         for (int f = 0; f < F; ++f) qv[f] = q[(e * F + f) * NP + n];
-        Pde<PDE>::fdir(dir, qv, o);
+        Pde<PDE>::fdir(0, qv, o);
 #pragma unroll
-        for (int f = 0; f < F; ++f) out[(e * F + f) * NP + n] = o[f];
+        for (int f = 0; f < F; ++f) fx[(e * F + f) * NP + n] = o[f];
+        Pde<PDE>::fdir(1, qv, o);
+#pragma unroll
+        for (int f = 0; f < F; ++f) fy[(e * F + f) * NP + n] = o[f];
+        Pde<PDE>::fdir(2, qv, o);
+#pragma unroll
+        for (int f = 0; f < F; ++f) fz[(e * F + f) * NP + n] = o[f];
     }
 }
 
@@ -316,11 +327,11 @@ int bb_interface_flux(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, con
 
 }  // extern "C"
 
-int bb_pde_fdir(bb_ctx *ctx, int pde, int dir, int N, int64_t K, const double *q, double *out) {
+int bb_pde_fdir3(bb_ctx *ctx, int pde, int N, int64_t K, const double *q, double *fx, double *fy, double *fz) {
     int NP = np3(N);
     unsigned g = grid_cap(K * NP);
-    if (pde == BB_PDE_WAVE) k_pde_fdir<BB_PDE_WAVE><<<g, 256, 0, ctx->stream>>>(dir, NP, q, out, K);
-    else k_pde_fdir<BB_PDE_MAXWELL><<<g, 256, 0, ctx->stream>>>(dir, NP, q, out, K);
+    if (pde == BB_PDE_WAVE) k_pde_fdir3<BB_PDE_WAVE><<<g, 256, 0, ctx->stream>>>(NP, q, fx, fy, fz, K);
+    else k_pde_fdir3<BB_PDE_MAXWELL><<<g, 256, 0, ctx->stream>>>(NP, q, fx, fy, fz, K);
     BB_LAUNCH_CHECK(ctx);
     return BB_OK;
 }

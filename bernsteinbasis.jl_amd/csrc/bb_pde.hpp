@@ -16,7 +16,7 @@ template <int PDE> struct Pde;
 template <> struct Pde<BB_PDE_WAVE> {
     static constexpr int F = 4;
     static constexpr bool plus = false;  // du -= sum geo * D f
-    // flux vector fx/fy/fz (dir 0/1/2) applied to a state
+    // flux vector fx/fy/fz (dir 0/1/2) applied to a state; call with a constant dir
     template <typename T> __device__ static void fdir(int dir, const T *q, T *o) {
         o[0] = dir == 0 ? q[1] : dir == 1 ? q[2] : q[3];
         o[1] = dir == 0 ? q[0] : T(0);

@@ -166,8 +166,7 @@ static int plain_rhs(bb_solver *s, const double *q) {
     if ((rc = bb_interface_flux(ctx, s->pde, s->flux_kind, N, K, s->uM, lf, s->mapP, s->bcn, s->sign, s->nx, s->ny,
                                 s->nz, s->flux, lf)))
         return rc;
-    for (int dir = 0; dir < 3; ++dir)
-        if ((rc = bb_pde_fdir(ctx, s->pde, dir, N, K, q, s->fu[dir]))) return rc;
+    if ((rc = bb_pde_fdir3(ctx, s->pde, N, K, q, s->fu[0], s->fu[1], s->fu[2]))) return rc;
     for (int c = 0; c < 3; ++c)
         for (int ax = 0; ax < 3; ++ax)
             if ((rc = bb_deriv_apply(ctx, ax, N, F, K, BB_F64, s->fu[c], lv, s->d9[3 * c + ax], lv))) return rc;
