@@ -49,6 +49,7 @@ SIGS = {
     "bb_memcpy_d2h": (i32, [vp, vp, vp, C.c_size_t]),
     "bb_memcpy_d2d": (i32, [vp, vp, vp, C.c_size_t]),
     "bb_fill_zero": (i32, [vp, vp, C.c_size_t]),
+    "bb_stream_copy": (i32, [vp, vp, vp, C.c_size_t]),
     "bb_synchronize": (i32, [vp]),
     "bb_timer_start": (i32, [vp]),
     "bb_timer_stop": (i32, [vp, C.POINTER(C.c_float)]),

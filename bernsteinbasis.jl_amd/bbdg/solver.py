@@ -89,12 +89,18 @@ class Solver:
         self.ctx.check(B.lib().bb_solver_graph_launch(self.h))
 
     def autotune(self, prec="f64"):
+        """time every compiled tiling of the fused kernel; returns the chosen tiling code
+        (elements per workgroup + 16 x lift organisation)"""
         e = C.c_int()
         self.ctx.check(B.lib().bb_solver_autotune(self.h, PREC[prec], C.byref(e)))
         return e.value
 
-    def set_tuning(self, prec, elems_per_block):
-        self.ctx.check(B.lib().bb_solver_set_tuning(self.h, PREC[prec], int(elems_per_block)))
+    def set_tuning(self, prec, tiling):
+        self.ctx.check(B.lib().bb_solver_set_tuning(self.h, PREC[prec], int(tiling)))
+
+    @staticmethod
+    def tiling_code(elems_per_block, lift="gather"):
+        return int(elems_per_block) + 16 * {"gather": 0, "pingpong": 1}[lift]
 
     def stage_bytes(self, prec="f64"):
         b, d = C.c_double(), C.c_double()

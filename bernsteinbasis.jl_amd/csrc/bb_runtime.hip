@@ -131,7 +131,28 @@ __global__ void bb_fill_zero_bytes(uint8_t *p, size_t n) {
     if (i < n) p[i] = 0;
 }
 
+// dst = src in 16-byte words: the bandwidth reference for the roofline (read + write of
+// `bytes` each) and the calibration kernel for the HBM byte counters
+__global__ void bb_stream_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 extern "C" {
+
+int bb_stream_copy(bb_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return BB_ERR_NULL;
+    if ((((uintptr_t)dst | (uintptr_t)src | bytes) & 15u) != 0)
+        return bb_fail(ctx, BB_ERR_ARG, "stream copy needs 16-byte aligned pointers and sizes");
+    const size_t n = bytes / 16;
+    if (!n) return BB_OK;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 1024 * 256) blocks = 1024 * 256;
+    bb_stream_copy_kernel<<<(unsigned)blocks, 256, 0, ctx->stream>>>((const uint4 *)src, (uint4 *)dst, n);
+    BB_LAUNCH_CHECK(ctx);
+    return BB_OK;
+}
 
 int bb_fill_zero(bb_ctx *ctx, void *dst, size_t bytes) {
     if (!ctx || (bytes && !dst)) return BB_ERR_NULL;

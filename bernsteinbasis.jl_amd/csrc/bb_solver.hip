@@ -37,7 +37,7 @@ struct bb_solver {
     int cur = 0;
     double *res64 = nullptr;
     float *res32 = nullptr;
-    int eb[2] = {0, 0};  // fused elements per block, per precision (0 = not chosen)
+    int tiling[2] = {0, 0};  // fused-kernel tiling code per precision (0 = not chosen)
     // graphs
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int g_path = -1, g_start = 0, g_flip = 0;
@@ -46,6 +46,14 @@ struct bb_solver {
 static StageLaunchFn stage_fn(int pde, int prec) {
     if (pde == BB_PDE_WAVE) return prec == BB_PREC_MIXED ? stage_launch_wave_mixed : stage_launch_wave_f64;
     return prec == BB_PREC_MIXED ? stage_launch_maxwell_mixed : stage_launch_maxwell_f64;
+}
+
+// is a tiling compiled for this (pde, prec, N)?  a launch over zero elements is a no-op
+// grid that only resolves the template
+static bool stage_valid(bb_solver *s, int prec, int tiling) {
+    StageArgs a{};
+    a.K = 0;
+    return stage_fn(s->pde, prec)(s->N, tiling, a, nullptr, true) == BB_OK;
 }
 
 static size_t state_words(const bb_solver *s) { return (size_t)s->K * s->F * s->NP; }
@@ -66,24 +74,26 @@ static int ensure_res(bb_solver *s, int prec) {
     return BB_OK;
 }
 
-// default elements per block: the fewest elements that give >= 192 owned threads
-static int default_eb(const bb_solver *s, int prec) {
-    int best = 0;
-    for (int e : kStageEB) {
-        StageArgs a{};
-        (void)a;
-        if (e * s->NP > 1024) break;
-        best = e;
-        if (e * s->NP >= 192) break;
+// default tiling: gather lift organisation, the fewest elements that give >= 192 owned
+// threads, falling back to the ping-pong organisation where the gather one does not fit
+static int default_tiling(bb_solver *s, int prec) {
+    for (int lm : kStageLM) {
+        int best = 0;
+        for (int e : kStageEB) {
+            if (e * s->NP > 1024) break;
+            best = e;
+            if (e * s->NP >= 192) break;
+        }
+        for (int e = best; e >= 1; e /= 2)
+            if (stage_valid(s, prec, tiling_code(e, lm))) return tiling_code(e, lm);
     }
-    (void)prec;
-    return best ? best : 1;
+    return 0;
 }
 
-static int launch_stage(bb_solver *s, int prec, int eb, const StageArgs &a) {
-    int rc = stage_fn(s->pde, prec)(s->N, eb, a, s->ctx->stream);
+static int launch_stage(bb_solver *s, int prec, int tiling, const StageArgs &a) {
+    int rc = stage_fn(s->pde, prec)(s->N, tiling, a, s->ctx->stream, false);
     if (rc == BB_ERR_UNSUPPORTED)
-        return bb_fail(s->ctx, rc, "no fused kernel for N=%d with %d elements per block", s->N, eb);
+        return bb_fail(s->ctx, rc, "no fused kernel for N=%d with tiling %d", s->N, tiling);
     if (rc) return bb_fail(s->ctx, rc, "fused stage launch failed: %s", hipGetErrorString(hipGetLastError()));
     return BB_OK;
 }
@@ -192,7 +202,7 @@ static int enqueue_steps(bb_solver *s, int path, int prec, double dt, int nsteps
                 a.cb = kB[stage];
                 a.dt = dt;
                 a.first = stage == 0;
-                if ((rc = launch_stage(s, prec, s->eb[prec], a))) return rc;
+                if ((rc = launch_stage(s, prec, s->tiling[prec], a))) return rc;
                 *cur ^= 1;
             }
         }
@@ -207,7 +217,8 @@ static int check_path(bb_solver *s, int path, int prec) {
     int rc;
     if ((rc = ensure_res(s, path == BB_PATH_PLAIN ? BB_PREC_F64 : prec))) return rc;
     if (path == BB_PATH_PLAIN) return ensure_plain(s);
-    if (!s->eb[prec]) s->eb[prec] = default_eb(s, prec);
+    if (!s->tiling[prec]) s->tiling[prec] = default_tiling(s, prec);
+    if (!s->tiling[prec]) return bb_fail(s->ctx, BB_ERR_UNSUPPORTED, "no fused kernel fits N=%d", s->N);
     return BB_OK;
 }
 
@@ -338,7 +349,7 @@ int bb_solver_rhs(bb_solver *s, int path, int prec, double *du_host) {
     a.rhs_only = 1;
     a.first = 1;
     a.dt = 1.0;
-    if ((rc = launch_stage(s, prec, s->eb[prec], a))) return rc;
+    if ((rc = launch_stage(s, prec, s->tiling[prec], a))) return rc;
     if (prec == BB_PREC_F64) {
         BB_HIP(ctx, hipMemcpyAsync(du_host, s->res64, vw * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         BB_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -400,18 +411,17 @@ int bb_solver_graph_launch(bb_solver *s) {
     return BB_OK;
 }
 
-int bb_solver_set_tuning(bb_solver *s, int prec, int elems_per_block) {
+int bb_solver_set_tuning(bb_solver *s, int prec, int tiling) {
     if (!s) return BB_ERR_NULL;
     if (prec != BB_PREC_F64 && prec != BB_PREC_MIXED) return bb_fail(s->ctx, BB_ERR_ARG, "prec %d", prec);
-    bool ok = false;
-    for (int e : kStageEB) ok |= e == elems_per_block;
-    if (!ok) return bb_fail(s->ctx, BB_ERR_ARG, "elements per block %d not compiled", elems_per_block);
-    s->eb[prec] = elems_per_block;
+    if (!stage_valid(s, prec, tiling))
+        return bb_fail(s->ctx, BB_ERR_ARG, "tiling %d not compiled for N=%d", tiling, s->N);
+    s->tiling[prec] = tiling;
     drop_graphs(s);
     return BB_OK;
 }
 
-int bb_solver_autotune(bb_solver *s, int prec, int *elems_per_block) {
+int bb_solver_autotune(bb_solver *s, int prec, int *tiling) {
     if (!s) return BB_ERR_NULL;
     int rc = check_path(s, BB_PATH_FUSED, prec);
     if (rc) return rc;
@@ -424,26 +434,28 @@ int bb_solver_autotune(bb_solver *s, int prec, int *elems_per_block) {
     a.dt = 0.0;
     a.first = 0;
     float best = 1e30f;
-    int best_eb = 0;
-    for (int e : kStageEB) {
-        if (e * s->NP > 1024) break;
-        if (stage_fn(s->pde, prec)(s->N, e, a, ctx->stream) != BB_OK) continue;  // not compiled: skip
-        float ms = 0.f;
-        BB_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        BB_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        for (int r = 0; r < 5; ++r) stage_fn(s->pde, prec)(s->N, e, a, ctx->stream);
-        BB_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-        BB_HIP(ctx, hipEventSynchronize(ctx->ev1));
-        BB_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        if (ms < best) {
-            best = ms;
-            best_eb = e;
+    int best_t = 0;
+    for (int lm : kStageLM)
+        for (int e : kStageEB) {
+            const int t = tiling_code(e, lm);
+            if (!stage_valid(s, prec, t)) continue;
+            float ms = 0.f;
+            if ((rc = launch_stage(s, prec, t, a))) return rc;  // warm
+            BB_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+            for (int r = 0; r < 5; ++r)
+                if ((rc = launch_stage(s, prec, t, a))) return rc;
+            BB_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            BB_HIP(ctx, hipEventSynchronize(ctx->ev1));
+            BB_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ms < best) {
+                best = ms;
+                best_t = t;
+            }
         }
-    }
-    if (!best_eb) return bb_fail(ctx, BB_ERR_UNSUPPORTED, "no fused kernel compiled for N=%d", s->N);
-    s->eb[prec] = best_eb;
+    if (!best_t) return bb_fail(ctx, BB_ERR_UNSUPPORTED, "no fused kernel compiled for N=%d", s->N);
+    s->tiling[prec] = best_t;
     drop_graphs(s);
-    if (elems_per_block) *elems_per_block = best_eb;
+    if (tiling) *tiling = best_t;
     return BB_OK;
 }
 

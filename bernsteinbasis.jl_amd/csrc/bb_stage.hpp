@@ -41,14 +41,27 @@ struct StageArgs {
     double sign[6];
 };
 
-template <int N, int PDE, bool MIXED, int EB> struct StageCfg {
+// Lift organisation inside the stage kernel:
+//  GATHER   : every chain vector of every face is kept in LDS (slab s of face f at the
+//             volume offset of slab s, i.e. the reference's G1..G3 arrays without the l_s
+//             weights); each thread then gathers its node's four face values once, in the
+//             reference's order G4 + G1 + G2 + G3.  LDS: 4 F Np + 4 F Nfp values.
+//  PINGPONG : only the current and next chain vectors are kept; after every step each
+//             thread adds the values that land on its node.  LDS: ~8 F Nfp values, but the
+//             per-step accumulation diverges across the lanes of a wave.
+enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1 };
+
+template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct StageCfg {
     using T = std::conditional_t<MIXED, float, double>;
     static constexpr int F = Pde<PDE>::F;
     static constexpr int NP = np3(N), NFP = nfp2(N);
     static constexpr int NT = ((EB * NP + 63) / 64) * 64;
-    static constexpr int R1 = (EB * F * NP > EB * F * 4 * NFP) ? EB * F * NP : EB * F * 4 * NFP;
+    // PINGPONG: s1 = state, then chain A; s2 = flux, then chain B
+    // GATHER:   s1 = state, then all chain vectors (EB F 4 Np); s2 = flux
+    static constexpr int R1 = LM == LIFT_GATHER ? EB * F * 4 * NP
+                              : ((EB * F * NP > EB * F * 4 * NFP) ? EB * F * NP : EB * F * 4 * NFP);
     static constexpr int R2 = EB * F * 4 * NFP;
-    static constexpr size_t lds = (size_t)(R1 + R2) * sizeof(T) + (size_t)NP * 2;
+    static constexpr size_t lds = (size_t)(R1 + R2) * sizeof(T) + (size_t)NP * 2 + (N + 1) * 8;
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024;
 };
 
@@ -57,15 +70,16 @@ template <int N> struct FaceSteps {
     int idx[4];
 };
 
-template <int N, int PDE, bool MIXED, int EB>
-__global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB>::NT)) void k_stage(StageArgs A) {
-    using Cfg = StageCfg<N, PDE, MIXED, EB>;
+template <int N, int PDE, bool MIXED, int EB, int LM>
+__global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage(StageArgs A) {
+    using Cfg = StageCfg<N, PDE, MIXED, EB, LM>;
     using T = typename Cfg::T;
     using P = Pde<PDE>;
     constexpr int F = Cfg::F, NP = Cfg::NP, NFP = Cfg::NFP, NT = Cfg::NT;
     __shared__ T s1[Cfg::R1];
     __shared__ T s2[Cfg::R2];
     __shared__ uint16_t dec[NP];
+    __shared__ double lw[N + 1];  // l_0 = 1, l_1 .. l_N (src/constants.jl:10-12)
 
     const int tid = threadIdx.x;
     const int64_t e0 = (int64_t)blockIdx.x * EB;
@@ -81,6 +95,11 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB>::NT)) void k_stage(Sta
         int a, b;
         decode2(d, u, a, b);
         dec[t] = (uint16_t)(a | (b << 8));
+    }
+    if (tid <= N) {
+#pragma unroll
+        for (int s = 0; s <= N; ++s)
+            if (tid == s) lw[s] = lj(N, s);
     }
     // ---- load the element's state (coalesced: consecutive threads, consecutive nodes)
     double qv[F];
@@ -143,49 +162,82 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB>::NT)) void k_stage(Sta
     }
     __syncthreads();
 
-    // ---- L0 on all chains (element, field, face): s2 -> s1
-    for (int t = tid; t < ne * F * 4 * NFP; t += NT) {
-        const int q = t / NFP, beta = t % NFP;
-        const int ab = dec[np3(N - 1) + beta];
-        s1[q * NFP + beta] = l0_apply<N, T>(s2 + q * NFP, ab & 255, ab >> 8);
-    }
-    // where each face's chain lands on this thread's node
+    // where each face's chain lands on this thread's node: face f1..f4 -> (slab, index)
     const int l = N - i - j - k;
     const int st[4] = {j, l, i, k};
     const int ix[4] = {lin2(N - j, i, k), lin2(N - l, j, k), lin2(N - i, j, k), lin2(N - k, i, j)};
     double acc[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) acc[f] = 0.0;
-    __syncthreads();
+    const int order[4] = {3, 0, 1, 2};  // face 4 first, then G1, G2, G3 (BernsteinLift.jl:202-217)
 
-    auto accumulate = [&](int s, const T *buf, double w) {
-        if (!own) return;
-        const int order[4] = {3, 0, 1, 2};  // face 4 first, then G1, G2, G3 (BernsteinLift.jl:202-217)
-#pragma unroll
-        for (int o = 0; o < 4; ++o) {
-            const int ff = order[o];
-            if (st[ff] == s) {
-#pragma unroll
-                for (int f = 0; f < F; ++f) acc[f] += w * (double)buf[((el * F + f) * 4 + ff) * NFP + ix[ff]];
-            }
-        }
-    };
-    accumulate(0, s1, 1.0);
-    T *cur = s1, *nxt = s2;
-#pragma unroll
-    for (int s = 1; s <= N; ++s) {
-        const int dd = N - s;
-        const T inv = T(1) / T(dd + 1);
-        for (int t = tid; t < ne * F * 4 * nfp2(dd); t += NT) {
-            const int q = t / nfp2(dd), beta = t % nfp2(dd);
-            const int ab = dec[np3(dd - 1) + beta];
-            nxt[q * NFP + beta] = reduce_apply<T>(cur + q * NFP, dd, ab & 255, ab >> 8, inv);
+    if constexpr (LM == LIFT_GATHER) {
+        // ---- L0 on all chains (element, field, face): s2 -> slab 0 of s1
+        for (int t = tid; t < ne * F * 4 * NFP; t += NT) {
+            const int q = t / NFP, beta = t % NFP;
+            const int ab = dec[np3(N - 1) + beta];
+            s1[q * NP + beta] = l0_apply<N, T>(s2 + q * NFP, ab & 255, ab >> 8);
         }
         __syncthreads();
-        accumulate(s, nxt, lj(N, s));
-        T *tmp = cur;
-        cur = nxt;
-        nxt = tmp;
+        // ---- reduction chain: slab s-1 (degree N-s+1) -> slab s (degree N-s)
+#pragma unroll
+        for (int s = 1; s <= N; ++s) {
+            const int dd = N - s;
+            const T inv = T(1) / T(dd + 1);
+            for (int t = tid; t < ne * F * 4 * nfp2(dd); t += NT) {
+                const int q = t / nfp2(dd), beta = t % nfp2(dd);
+                const int ab = dec[np3(dd - 1) + beta];
+                s1[q * NP + tet(N, s) + beta] = reduce_apply<T>(s1 + q * NP + tet(N, s - 1), dd, ab & 255, ab >> 8, inv);
+            }
+            __syncthreads();
+        }
+        // ---- each node gathers l_slab * E from its four faces
+        if (own) {
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                const int ff = order[o];
+                const double w = lw[st[ff]];
+                const int off = tet(N, st[ff]) + ix[ff];
+#pragma unroll
+                for (int f = 0; f < F; ++f) acc[f] += w * (double)s1[((el * F + f) * 4 + ff) * NP + off];
+            }
+        }
+    } else {
+        // ---- L0 on all chains: s2 -> s1
+        for (int t = tid; t < ne * F * 4 * NFP; t += NT) {
+            const int q = t / NFP, beta = t % NFP;
+            const int ab = dec[np3(N - 1) + beta];
+            s1[q * NFP + beta] = l0_apply<N, T>(s2 + q * NFP, ab & 255, ab >> 8);
+        }
+        __syncthreads();
+        auto accumulate = [&](int s, const T *buf, double w) {
+            if (!own) return;
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                const int ff = order[o];
+                if (st[ff] == s) {
+#pragma unroll
+                    for (int f = 0; f < F; ++f) acc[f] += w * (double)buf[((el * F + f) * 4 + ff) * NFP + ix[ff]];
+                }
+            }
+        };
+        accumulate(0, s1, 1.0);
+        T *cur = s1, *nxt = s2;
+#pragma unroll
+        for (int s = 1; s <= N; ++s) {
+            const int dd = N - s;
+            const T inv = T(1) / T(dd + 1);
+            for (int t = tid; t < ne * F * 4 * nfp2(dd); t += NT) {
+                const int q = t / nfp2(dd), beta = t % nfp2(dd);
+                const int ab = dec[np3(dd - 1) + beta];
+                nxt[q * NFP + beta] = reduce_apply<T>(cur + q * NFP, dd, ab & 255, ab >> 8, inv);
+            }
+            __syncthreads();
+            accumulate(s, nxt, lj(N, s));
+            T *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+        }
     }
 
     // ---- epilogue: rhs = (lift + volume) / J, then the RK stage
@@ -210,39 +262,53 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB>::NT)) void k_stage(Sta
 }
 
 // launch one stage; returns BB_ERR_UNSUPPORTED for an invalid (N, EB) combination
-using StageLaunchFn = int (*)(int N, int eb, const StageArgs &a, hipStream_t st);
-int stage_launch_wave_f64(int N, int eb, const StageArgs &a, hipStream_t st);
-int stage_launch_wave_mixed(int N, int eb, const StageArgs &a, hipStream_t st);
-int stage_launch_maxwell_f64(int N, int eb, const StageArgs &a, hipStream_t st);
-int stage_launch_maxwell_mixed(int N, int eb, const StageArgs &a, hipStream_t st);
+// launch one stage with the given tiling code (or, with probe = true, only report whether
+// that tiling is compiled); BB_ERR_UNSUPPORTED for a tiling that is not
+using StageLaunchFn = int (*)(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
+int stage_launch_wave_f64(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
+int stage_launch_wave_mixed(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
+int stage_launch_maxwell_f64(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
+int stage_launch_maxwell_mixed(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
 
 constexpr int kStageMaxN = 10;
-constexpr int kStageEB[] = {1, 2, 3, 4, 6, 8};
+// candidate tilings (elements per block, lift organisation), in autotuning order
+constexpr int kStageEB[] = {1, 2, 4};
+constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG};
+
+// tiling code: elements per block + 16 * lift organisation
+constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
 
 }  // namespace bb
 
 #define BB_STAGE_DEFINE(NAME, PDE, MIXED)                                                              \
     namespace bb {                                                                                     \
-    int NAME(int N, int eb, const StageArgs &a, hipStream_t st) {                                     \
+    int NAME(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe) {                     \
         const int64_t K = a.K;                                                                         \
         return dispatch_n<1, kStageMaxN>(N, [&](auto nc) -> int {                                      \
             constexpr int NN = decltype(nc)::value;                                                    \
             int rc = BB_ERR_UNSUPPORTED;                                                               \
-            auto one = [&](auto ebc) {                                                                 \
-                constexpr int E = decltype(ebc)::value;                                                \
-                if constexpr (StageCfg<NN, PDE, MIXED, E>::valid) {                                    \
-                    if (eb == E) {                                                                     \
-                        k_stage<NN, PDE, MIXED, E><<<grid1d(K, E), StageCfg<NN, PDE, MIXED, E>::NT, 0, st>>>(a); \
-                        rc = hipGetLastError() == hipSuccess ? BB_OK : BB_ERR_HIP;                     \
+            auto one = [&](auto ebc, auto lmc) {                                                       \
+                constexpr int E = decltype(ebc)::value, LMV = decltype(lmc)::value;                    \
+                using Cfg = StageCfg<NN, PDE, MIXED, E, LMV>;                                          \
+                if constexpr (Cfg::valid) {                                                            \
+                    if (tiling == tiling_code(E, LMV)) {                                               \
+                        if (probe) {                                                                   \
+                            rc = BB_OK;                                                                \
+                        } else {                                                                       \
+                            k_stage<NN, PDE, MIXED, E, LMV><<<grid1d(K, E), Cfg::NT, 0, st>>>(a);      \
+                            rc = hipGetLastError() == hipSuccess ? BB_OK : BB_ERR_HIP;                 \
+                        }                                                                              \
                     }                                                                                  \
                 }                                                                                      \
             };                                                                                         \
-            one(std::integral_constant<int, 1>{});                                                     \
-            one(std::integral_constant<int, 2>{});                                                     \
-            one(std::integral_constant<int, 3>{});                                                     \
-            one(std::integral_constant<int, 4>{});                                                     \
-            one(std::integral_constant<int, 6>{});                                                     \
-            one(std::integral_constant<int, 8>{});                                                     \
+            using G0 = std::integral_constant<int, LIFT_GATHER>;                                       \
+            using P1 = std::integral_constant<int, LIFT_PINGPONG>;                                     \
+            one(std::integral_constant<int, 1>{}, G0{});                                               \
+            one(std::integral_constant<int, 2>{}, G0{});                                               \
+            one(std::integral_constant<int, 4>{}, G0{});                                               \
+            one(std::integral_constant<int, 1>{}, P1{});                                               \
+            one(std::integral_constant<int, 2>{}, P1{});                                               \
+            one(std::integral_constant<int, 4>{}, P1{});                                               \
             return rc;                                                                                 \
         });                                                                                            \
     }                                                                                                  \

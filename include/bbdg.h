@@ -82,6 +82,10 @@ int bb_memcpy_h2d(bb_ctx *ctx, void *dst, const void *src_host, size_t bytes);
 int bb_memcpy_d2h(bb_ctx *ctx, void *dst_host, const void *src, size_t bytes);
 int bb_memcpy_d2d(bb_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bb_fill_zero(bb_ctx *ctx, void *dst, size_t bytes); /* kernel fill, capturable */
+/* dst = src by a plain copy kernel (16-byte aligned): the measured-bandwidth reference of
+ * the roofline and the calibration kernel of the HBM byte counters.  No reference
+ * counterpart. */
+int bb_stream_copy(bb_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bb_synchronize(bb_ctx *ctx);
 
 /* HIP-event timing on the context stream.  bb_timer_stop returns the milliseconds
@@ -181,10 +185,12 @@ int bb_solver_step(bb_solver *s, int path, int prec, double dt, int nsteps);
 /* capture nsteps steps into a hipGraph, then replay it with bb_solver_graph_launch */
 int bb_solver_graph_capture(bb_solver *s, int path, int prec, double dt, int nsteps);
 int bb_solver_graph_launch(bb_solver *s);
-/* pick elements-per-workgroup of the fused stage kernel by timing every candidate on
- * the current state (the state is restored); returns the choice */
-int bb_solver_autotune(bb_solver *s, int prec, int *elems_per_block);
-int bb_solver_set_tuning(bb_solver *s, int prec, int elems_per_block);
+/* Tiling of the fused stage kernel: code = elements per workgroup (1, 2, 4) + 16 x lift
+ * organisation (0: all chain vectors kept in LDS and gathered once per node; 1: two
+ * chain buffers, per-step accumulation -- smaller LDS).  bb_solver_autotune times every
+ * compiled tiling on the current state (which it leaves unchanged) and keeps the fastest. */
+int bb_solver_autotune(bb_solver *s, int prec, int *tiling);
+int bb_solver_set_tuning(bb_solver *s, int prec, int tiling);
 /* HBM bytes one fused stage moves by the algorithmic count (state in/out, residual
  * in/out, neighbour traces, geometry, connectivity) and the DOF count K*Np*F */
 int bb_solver_stage_bytes(bb_solver *s, int prec, double *bytes, double *dofs);
