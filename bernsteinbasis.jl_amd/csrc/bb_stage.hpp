@@ -65,10 +65,12 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct Stage
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024;
 };
 
-template <int N> struct FaceSteps {
-    int step[4];
-    int idx[4];
-};
+constexpr int kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    const unsigned q = nb / kXcds, r = nb % kXcds;
+    const unsigned x = b % kXcds, s = b / kXcds;
+    return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
 
 template <int N, int PDE, bool MIXED, int EB, int LM>
 __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage(StageArgs A) {
@@ -82,7 +84,11 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     __shared__ double lw[N + 1];  // l_0 = 1, l_1 .. l_N (src/constants.jl:10-12)
 
     const int tid = threadIdx.x;
-    const int64_t e0 = (int64_t)blockIdx.x * EB;
+    // Workgroups are dealt round-robin to the 8 XCDs (workgroup b -> XCD b mod 8), each
+    // with its own L2.  Remap so that XCD x processes one contiguous range of element
+    // blocks: face neighbours (mostly the same Kuhn cube or the next one) are then read
+    // by the same XCD while their owner's lines are still in its L2.
+    const int64_t e0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EB;
     const int ne = (int)min((int64_t)EB, A.K - e0);
     const bool own = tid < ne * NP;
     const int el = own ? tid / NP : 0;
@@ -171,24 +177,47 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     for (int f = 0; f < F; ++f) acc[f] = 0.0;
     const int order[4] = {3, 0, 1, 2};  // face 4 first, then G1, G2, G3 (BernsteinLift.jl:202-217)
 
-    if constexpr (LM == LIFT_GATHER) {
-        // ---- L0 on all chains (element, field, face): s2 -> slab 0 of s1
-        for (int t = tid; t < ne * F * 4 * NFP; t += NT) {
-            const int q = t / NFP, beta = t % NFP;
-            const int ab = dec[np3(N - 1) + beta];
-            s1[q * NP + beta] = l0_apply<N, T>(s2 + q * NFP, ab & 255, ab >> 8);
+    // chain storage: GATHER keeps every slab of every chain ([chain][Np], slab s at the
+    // volume offset tet(N,s)); PINGPONG keeps two [chain][Nfp] buffers.  Work items are
+    // (element, face, face node); the index and weight arithmetic is done once per item and
+    // reused for the F fields of the item.
+    constexpr int QS = LM == LIFT_GATHER ? NP : NFP;
+    // ---- L0 on every (element, face): flux s2 -> slab 0 (buffer s1)
+    for (int t = tid; t < ne * 4 * NFP; t += NT) {
+        const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, beta = t % NFP;
+        const int ab = dec[np3(N - 1) + beta];
+        const L0Pt<T> pt = l0_prep<N, T>(ab & 255, ab >> 8);
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            const int q = (e2 * F + f) * 4 + phi;
+            s1[q * QS + beta] = l0_eval<T>(s2 + q * NFP, pt);
         }
-        __syncthreads();
-        // ---- reduction chain: slab s-1 (degree N-s+1) -> slab s (degree N-s)
+    }
+    __syncthreads();
+    // one reduction step of the chain, degree dd+1 -> dd, for every (element, face, node)
+    auto chain_step = [&](const T *src, T *dst, int src_off, int dst_off, int dd) {
+        const T inv = T(1) / T(dd + 1);
+        const int nf = nfp2(dd);
+        for (int t = tid; t < ne * 4 * nf; t += NT) {
+            const int e2 = t / (4 * nf), phi = (t / nf) & 3, beta = t % nf;
+            const int ab = dec[np3(dd - 1) + beta];
+            const int a = ab & 255, b = ab >> 8;
+            const int idx = lin2(dd + 1, a, b);
+            const int o1 = idx + 1, o2 = idx + (dd + 2 - b);
+            const T wa = T(a + 1), wb = T(b + 1), wc = T(dd + 1 - a - b);
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                const int q = (e2 * F + f) * 4 + phi;
+                const T *x = src + q * QS + src_off;
+                dst[q * QS + dst_off + beta] = fma(wa, x[o1], fma(wb, x[o2], wc * x[idx])) * inv;
+            }
+        }
+    };
+
+    if constexpr (LM == LIFT_GATHER) {
 #pragma unroll
         for (int s = 1; s <= N; ++s) {
-            const int dd = N - s;
-            const T inv = T(1) / T(dd + 1);
-            for (int t = tid; t < ne * F * 4 * nfp2(dd); t += NT) {
-                const int q = t / nfp2(dd), beta = t % nfp2(dd);
-                const int ab = dec[np3(dd - 1) + beta];
-                s1[q * NP + tet(N, s) + beta] = reduce_apply<T>(s1 + q * NP + tet(N, s - 1), dd, ab & 255, ab >> 8, inv);
-            }
+            chain_step(s1, s1, tet(N, s - 1), tet(N, s), N - s);
             __syncthreads();
         }
         // ---- each node gathers l_slab * E from its four faces
@@ -203,13 +232,6 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
             }
         }
     } else {
-        // ---- L0 on all chains: s2 -> s1
-        for (int t = tid; t < ne * F * 4 * NFP; t += NT) {
-            const int q = t / NFP, beta = t % NFP;
-            const int ab = dec[np3(N - 1) + beta];
-            s1[q * NFP + beta] = l0_apply<N, T>(s2 + q * NFP, ab & 255, ab >> 8);
-        }
-        __syncthreads();
         auto accumulate = [&](int s, const T *buf, double w) {
             if (!own) return;
 #pragma unroll
@@ -225,13 +247,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
         T *cur = s1, *nxt = s2;
 #pragma unroll
         for (int s = 1; s <= N; ++s) {
-            const int dd = N - s;
-            const T inv = T(1) / T(dd + 1);
-            for (int t = tid; t < ne * F * 4 * nfp2(dd); t += NT) {
-                const int q = t / nfp2(dd), beta = t % nfp2(dd);
-                const int ab = dec[np3(dd - 1) + beta];
-                nxt[q * NFP + beta] = reduce_apply<T>(cur + q * NFP, dd, ab & 255, ab >> 8, inv);
-            }
+            chain_step(cur, nxt, 0, 0, N - s);
             __syncthreads();
             accumulate(s, nxt, lj(N, s));
             T *tmp = cur;
@@ -242,12 +258,12 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
 
     // ---- epilogue: rhs = (lift + volume) / J, then the RK stage
     if (own) {
-        const double Je = A.J[eg];
+        const double invJ = 1.0 / A.J[eg];
         using TR = T;
         TR *res = reinterpret_cast<TR *>(A.res);
 #pragma unroll
         for (int f = 0; f < F; ++f) {
-            const double rhs = (acc[f] + (double)vol[f]) / Je;
+            const double rhs = (acc[f] + (double)vol[f]) * invJ;
             const int64_t g = (eg * F + f) * NP + n;
             if (A.rhs_only) {
                 res[g] = TR(rhs);

@@ -96,6 +96,44 @@ template <int N, typename T> __device__ __forceinline__ T l0_apply(const T *x, i
     return T(0.5) * s;
 }
 
+// The same L0 stencil split into an index/weight part (computed once per face node) and
+// an evaluation part (repeated for every field that shares the face node).
+template <typename T> struct L0Pt {
+    int i0, i1, i2, i3, i4, i5, i6;
+    T w0, w1, w2, w3, w4, w5, w6;
+};
+template <int N, typename T> __device__ __forceinline__ L0Pt<T> l0_prep(int a, int b) {
+    const int c = N - a - b;
+    const int idx = lin2(N, a, b);
+    const int rl = N - b + 1;
+    L0Pt<T> p;
+    p.i0 = idx;
+    p.i1 = b > 0 ? idx - rl : idx;
+    p.i2 = c > 0 ? idx + 1 : idx;
+    p.i3 = a > 0 ? idx + rl - 1 : idx;
+    p.i4 = c > 0 ? idx + rl : idx;
+    p.i5 = a > 0 ? idx - 1 : idx;
+    p.i6 = b > 0 ? idx - rl - 1 : idx;
+    p.w0 = T((a + 1) * (a + 1) + (b + 1) * (b + 1) + (c + 1) * (c + 1));
+    p.w1 = T((a + 1) * b);
+    p.w2 = T((a + 1) * c);
+    p.w3 = T((b + 1) * a);
+    p.w4 = T((b + 1) * c);
+    p.w5 = T((c + 1) * a);
+    p.w6 = T((c + 1) * b);
+    return p;
+}
+template <typename T> __device__ __forceinline__ T l0_eval(const T *x, const L0Pt<T> &p) {
+    T s = p.w0 * x[p.i0];
+    s += p.w1 * x[p.i1];
+    s += p.w2 * x[p.i2];
+    s += p.w3 * x[p.i3];
+    s += p.w4 * x[p.i4];
+    s += p.w5 * x[p.i5];
+    s += p.w6 * x[p.i6];
+    return T(0.5) * s;
+}
+
 // (E_{d}^{d+1})^T x: degree d+1 -> degree d at (a,b), reduction_multiply!
 // (src/BernsteinLift.jl:66-90): ((a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (d+1-a-b) x_{ab}) / (d+1)
 template <typename T> __device__ __forceinline__ T reduce_apply(const T *x, int d, int a, int b, T inv) {
