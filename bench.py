@@ -190,7 +190,7 @@ def main():
     # first stage reads no residual; average the algorithmic bytes over the five stages
     w = float(dofs)
     sres = 4.0
-    geo_b = m.K * (9 * 8 + 8 + 12 * 8 + 4 * 8)
+    geo_b = m.K * (9 * 8 + 8 + 20 * 8 + 4 * 8)
     step_bytes = 5 * (w * 16 + w * sres + geo_b) + 4 * w * sres
     achieved = step_bytes / 5 / (mx["ms_per_step"] * 1e-3 / 5) * 1e-9
     copy_gbs = stream_bandwidth(s.ctx)

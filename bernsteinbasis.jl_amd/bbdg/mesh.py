@@ -96,14 +96,28 @@ def geometric_factors(vx: np.ndarray):
     return geo, J, nJ
 
 
-def kuhn_mesh(n: int, jitter: float = 0.1, seed: int | None = 0, periodic: bool = True) -> TetMesh:
+def morton3(x, y, z):
+    """interleave the bits of three non-negative integers (x lowest)"""
+    code = np.zeros_like(np.asarray(x, dtype=np.int64))
+    for bit in range(21):
+        for d, v in enumerate((x, y, z)):
+            code |= ((np.asarray(v, dtype=np.int64) >> bit) & 1) << (3 * bit + d)
+    return code
+
+
+def kuhn_mesh(n: int, jitter: float = 0.1, seed: int | None = 0, periodic: bool = True,
+              order: str = "morton") -> TetMesh:
     """Unit cube, n^3 cubes, each Kuhn-split into 6 tets (K = 6 n^3).
 
     Interior lattice vertices (all coordinates strictly inside (0,n)) are moved by
     i.i.d. U[-jitter*h, jitter*h] per coordinate from ``numpy.random.default_rng(seed)``
     drawn as one ((n-1)^3, 3) array in lattice order (x fastest).  Boundary vertices stay on
-    the lattice, so periodic images match exactly.  Element order: cube (cx fastest, then
-    cy, cz), then the 6 permutations of (x,y,z) in lexicographic order.
+    the lattice, so periodic images match exactly.
+
+    Element order: cubes along a Morton (Z-order) curve of (cx, cy, cz) -- ``order="morton"``,
+    so that face neighbours in every direction have nearby element numbers -- or
+    lexicographically (cx fastest, ``order="lex"``); within a cube, the 6 permutations of
+    (x,y,z) in lexicographic order.  The numbering changes no geometry.
     """
     if n < 1:
         raise ValueError("n must be >= 1")
@@ -112,6 +126,11 @@ def kuhn_mesh(n: int, jitter: float = 0.1, seed: int | None = 0, periodic: bool 
     K = 6 * ncube
     c = np.arange(ncube)
     cx, cy, cz = c % n, (c // n) % n, c // (n * n)
+    if order == "morton":
+        perm_c = np.argsort(morton3(cx, cy, cz), kind="stable")
+        cx, cy, cz = cx[perm_c], cy[perm_c], cz[perm_c]
+    elif order != "lex":
+        raise ValueError(f"unknown element order {order!r}")
     corner = np.stack([cx, cy, cz], axis=1)  # (ncube,3)
     lat = np.empty((K, 4, 3), dtype=np.int64)
     for t, perm in enumerate(itertools.permutations(range(3))):
@@ -138,7 +157,7 @@ def kuhn_mesh(n: int, jitter: float = 0.1, seed: int | None = 0, periodic: bool 
         geo, J, nJ = geometric_factors(vx)
     nbr, nbr_face, perm, bc = _connect(lat, n, periodic)
     return TetMesh(vx=vx, geo=geo, J=J, nJ=nJ, nbr=nbr, nbr_face=nbr_face, perm=perm, bc=bc, h=h,
-                   periodic=periodic, meta={"n": n, "jitter": jitter, "seed": seed})
+                   periodic=periodic, meta={"n": n, "jitter": jitter, "seed": seed, "order": order})
 
 
 def _connect(lat: np.ndarray, n: int, periodic: bool):

@@ -162,7 +162,9 @@ __global__ void k_interface_flux(int kind, int N, const double *__restrict__ uM,
 #pragma unroll
             for (int f = 0; f < F; ++f) d[f] = pp[(int64_t)f * lm.f] - um[f];
         }
-        Pde<PDE>::flux(kind, d, nx[idx], ny[idx], nz[idx], o);
+        const double n0 = nx[idx], n1 = ny[idx], n2 = nz[idx];
+        const double Jf = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        Pde<PDE>::flux(kind, d, n0, n1, n2, Jf, 1.0 / Jf, o);
         double *po = flux + e * lf.e + (int64_t)m * lf.n;
 #pragma unroll
         for (int f = 0; f < F; ++f) po[(int64_t)f * lf.f] = o[f];

@@ -32,8 +32,10 @@ template <> struct Pde<BB_PDE_WAVE> {
         else vol[0] -= gz;
     }
     // interface flux, examples/3d_wave.jl:45-54 (kind 0 verbatim; kind 1: every term
-    // carries Jf^1, i.e. d_p -> Jf d_p and (n.d) -> (n.d)/Jf in the penalty)
-    template <typename T> __device__ static void flux(int kind, const T *d, T nx, T ny, T nz, T *o) {
+    // carries Jf^1, i.e. d_p -> Jf d_p and (n.d) -> (n.d)/Jf in the penalty).  Jf = |nJ| and
+    // invJf = 1/Jf come from the caller (per face of an affine element).
+    template <typename T>
+    __device__ static void flux(int kind, const T *d, T nx, T ny, T nz, T Jf, T invJf, T *o) {
         T ndotdU = nx * d[1] + ny * d[2] + nz * d[3];
         if (kind == BB_FLUX_REFERENCE) {
             o[0] = T(0.5) * (d[0] - ndotdU);
@@ -41,8 +43,7 @@ template <> struct Pde<BB_PDE_WAVE> {
             o[2] = T(0.5) * ((ndotdU - d[0]) * ny);
             o[3] = T(0.5) * ((ndotdU - d[0]) * nz);
         } else {
-            T Jf = sqrt(nx * nx + ny * ny + nz * nz);
-            T un = ndotdU / Jf;
+            T un = ndotdU * invJf;
             o[0] = T(0.5) * (Jf * d[0] - ndotdU);
             o[1] = T(0.5) * ((un - d[0]) * nx);
             o[2] = T(0.5) * ((un - d[0]) * ny);
@@ -71,14 +72,13 @@ template <> struct Pde<BB_PDE_MAXWELL> {
         }
     }
     // examples/3d_maxwell.jl:189-207 (kind 1: penalties d -> Jf d, (n.d)n -> (n.d)n/Jf)
-    template <typename T> __device__ static void flux(int kind, const T *d, T nx, T ny, T nz, T *o) {
+    template <typename T>
+    __device__ static void flux(int kind, const T *d, T nx, T ny, T nz, T Jf, T invJf, T *o) {
         T ndH = nx * d[0] + ny * d[1] + nz * d[2];
         T ndE = nx * d[3] + ny * d[4] + nz * d[5];
-        T Jf = T(1);
         if (kind != BB_FLUX_REFERENCE) {
-            Jf = sqrt(nx * nx + ny * ny + nz * nz);
-            ndH = ndH / Jf;
-            ndE = ndE / Jf;
+            ndH = ndH * invJf;
+            ndE = ndE * invJf;
         }
         T fx[6], fy[6], fz[6];
         fdir(0, d, fx);

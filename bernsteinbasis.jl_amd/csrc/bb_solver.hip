@@ -1,5 +1,6 @@
 // bb_solver.hip -- device-resident mesh + state, the two RHS paths, LSRK45 stepping,
 // hipGraph capture of whole time steps, and the elements-per-workgroup autotuner.
+#include <math.h>
 #include <string.h>
 
 #include <algorithm>
@@ -23,8 +24,9 @@ struct bb_solver {
     int64_t K = 0;
     double sign[6] = {1, 1, 1, 1, 1, 1};
     // mesh
-    double *geo = nullptr, *J = nullptr, *nJ = nullptr;
+    double *geo = nullptr, *J = nullptr, *nJ = nullptr;  // nJ: [K][4][5] nxJ nyJ nzJ Jf 1/Jf
     int2 *conn = nullptr;
+    uint16_t *tab = nullptr;
     std::vector<int2> conn_host;
     std::vector<double> nJ_host;
     // plain-path mesh arrays (StartUpDG layout) and scratch, allocated on first use
@@ -104,6 +106,7 @@ static StageArgs stage_args(bb_solver *s, int prec) {
     a.J = s->J;
     a.nJ = s->nJ;
     a.conn = s->conn;
+    a.tab = s->tab;
     a.K = s->K;
     a.flux_kind = s->flux_kind;
     for (int f = 0; f < 6; ++f) a.sign[f] = s->sign[f];
@@ -276,14 +279,32 @@ int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, cons
     };
     hipError_t e;
     const size_t vw = state_words(s);
+    std::vector<double> nj5(K * 20);
+    for (int64_t t = 0; t < K * 4; ++t) {
+        const double *n = nJ_host + t * 3;
+        const double Jf = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (!(Jf > 0.0)) {
+            delete s;
+            return bb_fail(ctx, BB_ERR_ARG, "zero face normal at element %lld", (long long)(t / 4));
+        }
+        double *o = nj5.data() + t * 5;
+        o[0] = n[0];
+        o[1] = n[1];
+        o[2] = n[2];
+        o[3] = Jf;
+        o[4] = 1.0 / Jf;
+    }
+    const std::vector<uint16_t> tab = stage_tables(N);
     if ((e = hipMalloc(&s->geo, K * 9 * sizeof(double))) || (e = hipMalloc(&s->J, K * sizeof(double))) ||
-        (e = hipMalloc(&s->nJ, K * 12 * sizeof(double))) || (e = hipMalloc(&s->conn, K * 4 * sizeof(int2))) ||
+        (e = hipMalloc(&s->nJ, K * 20 * sizeof(double))) || (e = hipMalloc(&s->conn, K * 4 * sizeof(int2))) ||
+        (e = hipMalloc(&s->tab, tab.size() * sizeof(uint16_t))) ||
         (e = hipMalloc(&s->q[0], vw * sizeof(double))) || (e = hipMalloc(&s->q[1], vw * sizeof(double))))
         return fail(e);
     if ((e = hipMemcpy(s->geo, geo_host, K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->J, J_host, K * sizeof(double), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->nJ, nJ_host, K * 12 * sizeof(double), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->conn, conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)))
+        (e = hipMemcpy(s->nJ, nj5.data(), K * 20 * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->conn, conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->tab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice)))
         return fail(e);
     if ((rc = bb_fill_zero(ctx, s->q[0], vw * sizeof(double))) || (rc = bb_fill_zero(ctx, s->q[1], vw * sizeof(double)))) {
         bb_solver_destroy(s);
@@ -297,7 +318,7 @@ int bb_solver_destroy(bb_solver *s) {
     if (!s) return BB_ERR_NULL;
     hipStreamSynchronize(s->ctx->stream);
     drop_graphs(s);
-    void *ptrs[] = {s->geo, s->J, s->nJ, s->conn, s->mapP, s->bcn, s->nx, s->ny, s->nz, s->uM, s->flux, s->du,
+    void *ptrs[] = {s->geo, s->J, s->nJ, s->conn, s->tab, s->mapP, s->bcn, s->nx, s->ny, s->nz, s->uM, s->flux, s->du,
                     s->q[0], s->q[1], s->res64, s->res32};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -464,7 +485,7 @@ int bb_solver_stage_bytes(bb_solver *s, int prec, double *bytes, double *dofs) {
     const double w = (double)state_words(s);
     const double sres = prec == BB_PREC_MIXED ? 4.0 : 8.0;
     // q in + q out (FP64), residual in + out, per-element geometry / normals / connectivity
-    if (bytes) *bytes = w * (8.0 + 8.0 + 2.0 * sres) + (double)s->K * (9 * 8 + 8 + 12 * 8 + 4 * 8);
+    if (bytes) *bytes = w * (8.0 + 8.0 + 2.0 * sres) + (double)s->K * (9 * 8 + 8 + 20 * 8 + 4 * 8);
     if (dofs) *dofs = w;
     return BB_OK;
 }

@@ -19,6 +19,7 @@
 // faces' chains, where cancellation happens), so no LDS scatter and no atomics.
 #pragma once
 #include <type_traits>
+#include <vector>
 
 #include "bb_pde.hpp"
 #include "bb_stencil.hpp"
@@ -31,8 +32,9 @@ struct StageArgs {
     void *res;
     const double *geo;
     const double *J;
-    const double *nJ;
+    const double *nJ;      // [K][4][5]: nxJ nyJ nzJ Jf 1/Jf per face
     const int2 *conn;
+    const uint16_t *tab;   // index tables of degree N (stage_tables)
     int64_t K;
     double ca, cb, dt;
     int first;     // stage 1: res is not read (a_1 = 0)
@@ -61,9 +63,38 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct Stage
     static constexpr int R1 = LM == LIFT_GATHER ? EB * F * 4 * NP
                               : ((EB * F * NP > EB * F * 4 * NFP) ? EB * F * NP : EB * F * 4 * NFP);
     static constexpr int R2 = EB * F * 4 * NFP;
-    static constexpr size_t lds = (size_t)(R1 + R2) * sizeof(T) + (size_t)NP * 2 + (N + 1) * 8;
+    static constexpr int TAB = 2 * NP + 28 * NFP;  // ijk, dec, own face nodes, neighbour face nodes
+    static constexpr size_t lds = (size_t)(R1 + R2) * sizeof(T) + (size_t)TAB * 2 + (N + 1) * 8;
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024;
 };
+
+// Index tables of degree N for the stage kernel, built once on the host (uint16 each):
+//   [0, Np)            node n -> i | j << 4 | k << 8
+//   [Np, 2 Np)         entry t of a degree-d triangle (d = 0..N, at offset Np(d-1)) -> a | b << 8
+//   [2 Np, +4 Nfp)     own face f, node m -> volume index
+//   [.., +24 Nfp)      neighbour face g, permutation p, my node m -> neighbour's volume index
+inline std::vector<uint16_t> stage_tables(int N) {
+    const int NP = np3(N), NFP = nfp2(N);
+    std::vector<uint16_t> t(2 * NP + 28 * NFP);
+    int r = 0;
+    for (int k = 0; k <= N; ++k)
+        for (int j = 0; j <= N - k; ++j)
+            for (int i = 0; i <= N - j - k; ++i) t[r++] = (uint16_t)(i | (j << 4) | (k << 8));
+    for (int d = 0; d <= N; ++d)
+        for (int b = 0; b <= d; ++b)
+            for (int a = 0; a <= d - b; ++a) t[r++] = (uint16_t)(a | (b << 8));
+    for (int f = 0; f < 4; ++f)
+        for (int b = 0; b <= N; ++b)
+            for (int a = 0; a <= N - b; ++a) t[r++] = (uint16_t)face_vidx(N, f, a, b);
+    for (int g = 0; g < 4; ++g)
+        for (int p = 0; p < 6; ++p)
+            for (int b = 0; b <= N; ++b)
+                for (int a = 0; a <= N - b; ++a) {
+                    const int abc[3] = {a, b, N - a - b};
+                    t[r++] = (uint16_t)face_vidx(N, g, abc[perm_entry(p, 0)], abc[perm_entry(p, 1)]);
+                }
+    return t;
+}
 
 constexpr int kXcds = 8;
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
@@ -80,8 +111,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     constexpr int F = Cfg::F, NP = Cfg::NP, NFP = Cfg::NFP, NT = Cfg::NT;
     __shared__ T s1[Cfg::R1];
     __shared__ T s2[Cfg::R2];
-    __shared__ uint16_t dec[NP];
+    __shared__ uint16_t tab[Cfg::TAB];
     __shared__ double lw[N + 1];  // l_0 = 1, l_1 .. l_N (src/constants.jl:10-12)
+    const uint16_t *ijk = tab, *dec = tab + NP, *fvid = tab + 2 * NP, *nvid = tab + 2 * NP + 4 * NFP;
 
     const int tid = threadIdx.x;
     // Workgroups are dealt round-robin to the 8 XCDs (workgroup b -> XCD b mod 8), each
@@ -95,13 +127,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     const int n = own ? tid % NP : 0;
     const int64_t eg = e0 + el;
 
-    for (int t = tid; t < NP; t += NT) {
-        int d = 0, u = t;
-        while (u >= nfp2(d)) { u -= nfp2(d); ++d; }
-        int a, b;
-        decode2(d, u, a, b);
-        dec[t] = (uint16_t)(a | (b << 8));
-    }
+    for (int t = tid; t < Cfg::TAB; t += NT) tab[t] = A.tab[t];
     if (tid <= N) {
 #pragma unroll
         for (int s = 0; s <= N; ++s)
@@ -122,8 +148,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     T vol[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) vol[f] = T(0);
-    int i = 0, j = 0, k = 0;
-    decode3<N>(n, i, j, k);
+    const int pk = ijk[n];
+    const int i = pk & 15, j = (pk >> 4) & 15, k = pk >> 8;
     if (own) {
         const double *g = A.geo + eg * 9;
         const T g0 = T(g[0]), g1 = T(g[1]), g2 = T(g[2]), g3 = T(g[3]), g4 = T(g[4]), g5 = T(g[5]), g6 = T(g[6]),
@@ -142,18 +168,14 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     // ---- interface flux at every face node of the block's elements -> s2
     for (int t = tid; t < ne * 4 * NFP; t += NT) {
         const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, m = t % NFP;
-        const int ab = dec[np3(N - 1) + m];
-        const int a = ab & 255, b = ab >> 8, c = N - a - b;
-        const int vi = face_vidx(N, phi, a, b);
+        const int vi = fvid[phi * NFP + m];
         T um[F], d[F], o[F];
 #pragma unroll
         for (int f = 0; f < F; ++f) um[f] = s1[(e2 * F + f) * NP + vi];
         const int64_t ef = e0 + e2;
         const int2 cn = A.conn[ef * 4 + phi];
         if (((cn.y >> 5) & 7) == 0) {
-            const int nface = cn.y & 3, p = (cn.y >> 2) & 7;
-            const int abc[3] = {a, b, c};
-            const int nvi = face_vidx(N, nface, abc[perm_entry(p, 0)], abc[perm_entry(p, 1)]);
+            const int nvi = nvid[((cn.y & 3) * 6 + ((cn.y >> 2) & 7)) * NFP + m];
             const double *qn = A.q_in + (int64_t)cn.x * F * NP + nvi;
 #pragma unroll
             for (int f = 0; f < F; ++f) d[f] = T(qn[f * NP]) - um[f];
@@ -161,8 +183,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
 #pragma unroll
             for (int f = 0; f < F; ++f) d[f] = T(A.sign[f]) * um[f] - um[f];
         }
-        const double *nj = A.nJ + (ef * 4 + phi) * 3;
-        P::template flux<T>(A.flux_kind, d, T(nj[0]), T(nj[1]), T(nj[2]), o);
+        const double *nj = A.nJ + (ef * 4 + phi) * 5;
+        P::template flux<T>(A.flux_kind, d, T(nj[0]), T(nj[1]), T(nj[2]), T(nj[3]), T(nj[4]), o);
 #pragma unroll
         for (int f = 0; f < F; ++f) s2[((e2 * F + f) * 4 + phi) * NFP + m] = o[f];
     }

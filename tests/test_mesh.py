@@ -82,3 +82,19 @@ def test_interpolation_reproduces_polynomials():
     vals = basis.evaluate(N, q)
     np.testing.assert_allclose(vals[..., 0], X[..., 0] ** 2 * X[..., 1] - X[..., 2] ** 3, atol=1e-12)
     np.testing.assert_allclose(q[..., 1], 1.0, atol=1e-12)  # partition of unity
+
+
+def test_morton_order_is_a_renumbering():
+    """the Morton numbering permutes the same elements (same geometry, same volumes)"""
+    from bbdg import mesh
+
+    a = mesh.kuhn_mesh(8, 0.1, 3, order="lex")
+    b = mesh.kuhn_mesh(8, 0.1, 3, order="morton")
+    key = lambda m: np.round(m.vx.reshape(m.K, -1), 12)
+    ka = {tuple(r) for r in key(a)}
+    kb = {tuple(r) for r in key(b)}
+    assert ka == kb
+    np.testing.assert_allclose(np.sort(a.J), np.sort(b.J))
+    # more face neighbours lie within a short window of element numbers
+    win = lambda m: (np.abs(m.nbr - np.arange(m.K)[:, None])[m.bc == 0] < 200).mean()
+    assert win(b) > win(a)
