@@ -64,7 +64,8 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct Stage
                               : ((EB * F * NP > EB * F * 4 * NFP) ? EB * F * NP : EB * F * 4 * NFP);
     static constexpr int R2 = EB * F * 4 * NFP;
     static constexpr int TAB = 2 * NP + 28 * NFP;  // ijk, dec, own face nodes, neighbour face nodes
-    static constexpr size_t lds = (size_t)(R1 + R2) * sizeof(T) + (size_t)TAB * 2 + (N + 1) * 8;
+    static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
+    static constexpr size_t lds = (size_t)(R1 + R2) * sizeof(T) + (size_t)TAB16 * 16 + (N + 1) * 8;
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024;
 };
 
@@ -75,7 +76,7 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct Stage
 //   [.., +24 Nfp)      neighbour face g, permutation p, my node m -> neighbour's volume index
 inline std::vector<uint16_t> stage_tables(int N) {
     const int NP = np3(N), NFP = nfp2(N);
-    std::vector<uint16_t> t(2 * NP + 28 * NFP);
+    std::vector<uint16_t> t((2 * NP + 28 * NFP + 7) / 8 * 8, 0);  // padded to 16-byte words
     int r = 0;
     for (int k = 0; k <= N; ++k)
         for (int j = 0; j <= N - k; ++j)
@@ -111,7 +112,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     constexpr int F = Cfg::F, NP = Cfg::NP, NFP = Cfg::NFP, NT = Cfg::NT;
     __shared__ T s1[Cfg::R1];
     __shared__ T s2[Cfg::R2];
-    __shared__ uint16_t tab[Cfg::TAB];
+    __shared__ uint4 tab16[Cfg::TAB16];
+    const uint16_t *tab = reinterpret_cast<const uint16_t *>(tab16);
     __shared__ double lw[N + 1];  // l_0 = 1, l_1 .. l_N (src/constants.jl:10-12)
     const uint16_t *ijk = tab, *dec = tab + NP, *fvid = tab + 2 * NP, *nvid = tab + 2 * NP + 4 * NFP;
 
@@ -127,22 +129,51 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     const int n = own ? tid % NP : 0;
     const int64_t eg = e0 + el;
 
-    for (int t = tid; t < Cfg::TAB; t += NT) tab[t] = A.tab[t];
+    // ---- prologue: every global read the stage needs is issued here, before the first
+    // LDS work, so that its latency overlaps the volume phase:
+    //   index tables -> LDS (one 16-byte word per thread), the element state and the RK
+    //   residual of the owned node, and the neighbour traces of this thread's face nodes
+    for (int t = tid; t < Cfg::TAB16; t += NT) tab16[t] = reinterpret_cast<const uint4 *>(A.tab)[t];
     if (tid <= N) {
 #pragma unroll
         for (int s = 0; s <= N; ++s)
             if (tid == s) lw[s] = lj(N, s);
     }
-    // ---- load the element's state (coalesced: consecutive threads, consecutive nodes)
+    using TR = T;
+    TR *res = reinterpret_cast<TR *>(A.res);
     double qv[F];
+    TR rv[F];
     if (own) {
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             qv[f] = A.q_in[(eg * F + f) * NP + n];
-            s1[(el * F + f) * NP + n] = T(qv[f]);
+            rv[f] = (A.first || A.rhs_only) ? TR(0) : res[(eg * F + f) * NP + n];
         }
     }
-    __syncthreads();
+    __syncthreads();  // tables visible
+    // face-node work items of this thread (at most RF of them)
+    constexpr int RF = (EB * 4 * NFP + NT - 1) / NT;
+    T nb[RF][F];
+    int2 cnv[RF];
+#pragma unroll
+    for (int r = 0; r < RF; ++r) {
+        const int t = tid + r * NT;
+        if (t < ne * 4 * NFP) {
+            const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, m = t % NFP;
+            cnv[r] = A.conn[(e0 + e2) * 4 + phi];
+            if (((cnv[r].y >> 5) & 7) == 0) {
+                const int nvi = nvid[((cnv[r].y & 3) * 6 + ((cnv[r].y >> 2) & 7)) * NFP + m];
+                const double *qn = A.q_in + (int64_t)cnv[r].x * F * NP + nvi;
+#pragma unroll
+                for (int f = 0; f < F; ++f) nb[r][f] = T(qn[f * NP]);
+            }
+        }
+    }
+    if (own) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) s1[(el * F + f) * NP + n] = T(qv[f]);
+    }
+    __syncthreads();  // state visible
 
     // ---- volume term: 3 reference derivatives per field, geometric combine, PDE flux Jacobian
     T vol[F];
@@ -166,27 +197,27 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     }
 
     // ---- interface flux at every face node of the block's elements -> s2
-    for (int t = tid; t < ne * 4 * NFP; t += NT) {
-        const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, m = t % NFP;
-        const int vi = fvid[phi * NFP + m];
-        T um[F], d[F], o[F];
 #pragma unroll
-        for (int f = 0; f < F; ++f) um[f] = s1[(e2 * F + f) * NP + vi];
-        const int64_t ef = e0 + e2;
-        const int2 cn = A.conn[ef * 4 + phi];
-        if (((cn.y >> 5) & 7) == 0) {
-            const int nvi = nvid[((cn.y & 3) * 6 + ((cn.y >> 2) & 7)) * NFP + m];
-            const double *qn = A.q_in + (int64_t)cn.x * F * NP + nvi;
+    for (int r = 0; r < RF; ++r) {
+        const int t = tid + r * NT;
+        if (t < ne * 4 * NFP) {
+            const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, m = t % NFP;
+            const int vi = fvid[phi * NFP + m];
+            T um[F], d[F], o[F];
 #pragma unroll
-            for (int f = 0; f < F; ++f) d[f] = T(qn[f * NP]) - um[f];
-        } else {
+            for (int f = 0; f < F; ++f) um[f] = s1[(e2 * F + f) * NP + vi];
+            if (((cnv[r].y >> 5) & 7) == 0) {
 #pragma unroll
-            for (int f = 0; f < F; ++f) d[f] = T(A.sign[f]) * um[f] - um[f];
+                for (int f = 0; f < F; ++f) d[f] = nb[r][f] - um[f];
+            } else {
+#pragma unroll
+                for (int f = 0; f < F; ++f) d[f] = T(A.sign[f]) * um[f] - um[f];
+            }
+            const double *nj = A.nJ + ((e0 + e2) * 4 + phi) * 5;
+            P::template flux<T>(A.flux_kind, d, T(nj[0]), T(nj[1]), T(nj[2]), T(nj[3]), T(nj[4]), o);
+#pragma unroll
+            for (int f = 0; f < F; ++f) s2[((e2 * F + f) * 4 + phi) * NFP + m] = o[f];
         }
-        const double *nj = A.nJ + (ef * 4 + phi) * 5;
-        P::template flux<T>(A.flux_kind, d, T(nj[0]), T(nj[1]), T(nj[2]), T(nj[3]), T(nj[4]), o);
-#pragma unroll
-        for (int f = 0; f < F; ++f) s2[((e2 * F + f) * 4 + phi) * NFP + m] = o[f];
     }
     __syncthreads();
 
@@ -281,8 +312,6 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     // ---- epilogue: rhs = (lift + volume) / J, then the RK stage
     if (own) {
         const double invJ = 1.0 / A.J[eg];
-        using TR = T;
-        TR *res = reinterpret_cast<TR *>(A.res);
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             const double rhs = (acc[f] + (double)vol[f]) * invJ;
@@ -290,7 +319,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
             if (A.rhs_only) {
                 res[g] = TR(rhs);
             } else {
-                const double r = A.first ? A.dt * rhs : A.ca * (double)res[g] + A.dt * rhs;
+                const double r = A.first ? A.dt * rhs : A.ca * (double)rv[f] + A.dt * rhs;
                 const TR rs = TR(r);
                 res[g] = rs;
                 A.q_out[g] = qv[f] + A.cb * (double)rs;
@@ -299,7 +328,6 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     }
 }
 
-// launch one stage; returns BB_ERR_UNSUPPORTED for an invalid (N, EB) combination
 // launch one stage with the given tiling code (or, with probe = true, only report whether
 // that tiling is compiled); BB_ERR_UNSUPPORTED for a tiling that is not
 using StageLaunchFn = int (*)(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
