@@ -240,11 +240,13 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
         const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, beta = t % NFP;
         const int ab = dec[np3(N - 1) + beta];
         const L0Pt<T> pt = l0_prep<N, T>(ab & 255, ab >> 8);
+        // all F evaluations (their LDS reads) before any store: the stores target another
+        // buffer, but the compiler cannot prove it and would serialise the fields
+        T v[F];
 #pragma unroll
-        for (int f = 0; f < F; ++f) {
-            const int q = (e2 * F + f) * 4 + phi;
-            s1[q * QS + beta] = l0_eval<T>(s2 + q * NFP, pt);
-        }
+        for (int f = 0; f < F; ++f) v[f] = l0_eval<T>(s2 + ((e2 * F + f) * 4 + phi) * NFP, pt);
+#pragma unroll
+        for (int f = 0; f < F; ++f) s1[((e2 * F + f) * 4 + phi) * QS + beta] = v[f];
     }
     __syncthreads();
     // one reduction step of the chain, degree dd+1 -> dd, for every (element, face, node)
@@ -258,12 +260,18 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
             const int idx = lin2(dd + 1, a, b);
             const int o1 = idx + 1, o2 = idx + (dd + 2 - b);
             const T wa = T(a + 1), wb = T(b + 1), wc = T(dd + 1 - a - b);
+            // reads of all F fields first, then the stores (src and dst may be the same array)
+            T x1[F], x2[F], x0[F];
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                const int q = (e2 * F + f) * 4 + phi;
-                const T *x = src + q * QS + src_off;
-                dst[q * QS + dst_off + beta] = fma(wa, x[o1], fma(wb, x[o2], wc * x[idx])) * inv;
+                const T *x = src + ((e2 * F + f) * 4 + phi) * QS + src_off;
+                x1[f] = x[o1];
+                x2[f] = x[o2];
+                x0[f] = x[idx];
             }
+#pragma unroll
+            for (int f = 0; f < F; ++f)
+                dst[((e2 * F + f) * 4 + phi) * QS + dst_off + beta] = fma(wa, x1[f], fma(wb, x2[f], wc * x0[f])) * inv;
         }
     };
 
