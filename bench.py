@@ -193,6 +193,15 @@ def main():
     geo_b = m.K * (9 * 8 + 8 + 20 * 8 + 4 * 8)
     step_bytes = 5 * (w * 16 + w * sres + geo_b) + 4 * w * sres
     achieved = step_bytes / 5 / (mx["ms_per_step"] * 1e-3 / 5) * 1e-9
+    # HBM traffic of the chosen stage kernel, from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes over this workload (scaled by the 1 GiB copy calibration)
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "stage_maxwell_n9_k196608.json")
+    kname = f"k_stage<9, 1, true, {mx['tiling'] % 16}, {mx['tiling'] // 16}>"
+    if os.path.exists(prof):
+        rec = json.load(open(prof))["kernels"].get(kname)
+        if rec:
+            traffic = rec["fetch_bytes"] + rec["write_bytes"]
     copy_gbs = stream_bandwidth(s.ctx)
     log(f"stream copy: {copy_gbs:.0f} GB/s")
     cpu = None
@@ -227,8 +236,9 @@ def main():
         "per_rhs_value": mx["value"] * 5,
         "errors_vs_f64": {"steps": nsteps_err, "rel_l2_all": tot, "fields": per_field},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stage<9,maxwell,mixed>", "bytes_per_launch": step_bytes / 5,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/stage_maxwell_n9_k196608.json (rocprofv3 FETCH_SIZE x2.0, WRITE_SIZE x1.0)",
+                     "kernel": kname, "bytes_per_launch": step_bytes / 5,
                      "launch_us_graph": mx["ms_per_step"] * 1e3 / 5, "launch_us_eager": mx["stage_ms"] * 1e3,
                      "stream_copy_GBs": copy_gbs, "frac_of_stream_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
