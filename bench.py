@@ -101,6 +101,113 @@ def stream_bandwidth(ctx, gib=2.0, reps=10):
     return 2 * n / (ms * 1e-3) * 1e-9
 
 
+def operator_bench(ctx, reps=10):
+    """Operator microbenchmark: K=262144 tets at N=7 (Np=120, Nfp=36), F=4 fields, volume
+    coefficients and face data i.i.d. U[-1,1] (seed 2), geometric factors cycled from the
+    32^3x6 jittered mesh (seed 1).  Times the three derivative matvecs with geometric
+    factors (grad) and the 4-face lift, FP64 and FP32, SoA layout [K][F][n]."""
+    import bbdg
+    from bbdg import mesh
+
+    N, F, K = 7, 4, 262144
+    NP, NFP = 120, 36
+    m = mesh.kuhn_mesh(32, 0.1, 1, periodic=False)
+    geo_h = np.ascontiguousarray(m.geo[np.arange(K) % m.K])
+    rng = np.random.default_rng(2)
+    x_h = rng.uniform(-1, 1, (K, F, NP))
+    f_h = rng.uniform(-1, 1, (K, F, 4 * NFP))
+    geo = bbdg.DeviceArray.from_numpy(geo_h, ctx=ctx)
+    out = {}
+    for dt, name, sz in ((np.float64, "f64", 8), (np.float32, "f32", 4)):
+        x = bbdg.DeviceArray.from_numpy(x_h.astype(dt), ctx=ctx)
+        xv = x.view((NP, K, F), (1, F * NP, NP))
+        o = [bbdg.DeviceArray.empty((K, F, NP), dt, ctx=ctx) for _ in range(3)]
+        ov = [a.view(xv.shape, xv.strides) for a in o]
+        bbdg.grad_(ov[0], ov[1], ov[2], N, xv, geo)
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(reps):
+            bbdg.grad_(ov[0], ov[1], ov[2], N, xv, geo)
+        ms = ctx.timer_stop() / reps
+        b = K * (F * 4 * NP * sz + 9 * 8)
+        out[f"grad_{name}"] = {"us": ms * 1e3, "GBs": b / (ms * 1e-3) * 1e-9, "bytes": b,
+                               "dof_per_s": K * F * NP / (ms * 1e-3)}
+        for a in o:
+            a.free()
+        fd = bbdg.DeviceArray.from_numpy(f_h.astype(dt), ctx=ctx)
+        fv = fd.view((4 * NFP, K, F), (1, F * 4 * NFP, 4 * NFP))
+        lo = bbdg.DeviceArray.empty((K, F, NP), dt, ctx=ctx)
+        lv = lo.view((NP, K, F), (1, F * NP, NP))
+        L = bbdg.BernsteinLift(N, dt, "acc64")
+        bbdg.mul_(lv, L, fv)
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(reps):
+            bbdg.mul_(lv, L, fv)
+        ms = ctx.timer_stop() / reps
+        b = K * F * (4 * NFP + NP) * sz
+        out[f"lift_{name}"] = {"us": ms * 1e3, "GBs": b / (ms * 1e-3) * 1e-9, "bytes": b,
+                               "dof_per_s": K * F * NP / (ms * 1e-3)}
+        for a in (x, fd, lo):
+            a.free()
+    geo.free()
+    for k, v in out.items():
+        v["frac_of_8TBs"] = v["GBs"] / HBM_PEAK_GBS
+    return out
+
+
+def lift_study(ctx, K=65536, ld_cols=2048):
+    """Lift conditioning study: single-face lift, N in {8, 11, 14, 17}, K columns of face
+    coefficients i.i.d. N(0,1) (seed 4), and the same scaled per column by 10^U[-3,3].
+    Variants: FP64; FP32 in, FP32 chain, FP32 out ("fp32"); FP32 in, FP64 chain and
+    weighting ("acc64"); FP32 in, float-float chain ("twosum").  Errors are relative L2 over
+    all outputs against the FP64 GPU lift of the FP64 input (the reference algorithm in
+    Float64) and, to isolate the arithmetic, against the FP64 lift of the FP32-rounded
+    input.  The FP64 row is itself measured against the long-double restatement in the
+    oracle on the first ld_cols columns."""
+    import oracle as O
+    import bbdg
+
+    rows = []
+    rng = np.random.default_rng(4)
+    for N in (8, 11, 14, 17):
+        nfp, np_ = (N + 1) * (N + 2) // 2, (N + 1) * (N + 2) * (N + 3) // 6
+        base = rng.standard_normal((K, nfp))
+        scale = 10.0 ** rng.uniform(-3, 3, (K, 1))
+        for dist, x in (("normal", base), ("scaled", base * scale)):
+            L = bbdg.BernsteinLift(N)
+
+            def run(xh, mode, out_dt):
+                xd = bbdg.DeviceArray.from_numpy(xh, ctx=ctx)
+                xv = xd.view((nfp, K), (1, nfp))
+                od = bbdg.DeviceArray.empty((K, np_), out_dt, ctx=ctx)
+                ov = od.view((np_, K), (1, np_))
+                bbdg.face_lift_multiply_(ov, L, xv, mode=mode)
+                ctx.sync()
+                ctx.timer_start()
+                bbdg.face_lift_multiply_(ov, L, xv, mode=mode)
+                us = ctx.timer_stop() * 1e3
+                r = od.numpy().astype(np.float64)
+                xd.free()
+                od.free()
+                return r, us
+
+            ref, us64 = run(x, "plain", np.float64)
+            x32 = x.astype(np.float32)
+            ref32in, _ = run(x32.astype(np.float64), "plain", np.float64)
+            ld = O.face_lift_ld(N, x[:ld_cols])
+            nr = np.linalg.norm(ref)
+            rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            row = {"N": N, "dist": dist, "fp64": {"err_vs_long_double": rel(ref[:ld_cols], ld), "us": us64}}
+            for name, mode, odt in (("fp32", "plain", np.float32), ("acc64", "acc64", np.float64),
+                                    ("twosum", "twosum", np.float64)):
+                got, us = run(x32, mode, odt)
+                row[name] = {"err_vs_fp64": rel(got, ref), "err_vs_fp64_of_fp32_input": rel(got, ref32in), "us": us}
+            rows.append(row)
+            log(json.dumps(row))
+    return rows
+
+
 def cpu_baseline(n_sample, steps, threads):
     import oracle as O
     from bbdg import basis, mesh
@@ -132,6 +239,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-ops", action="store_true", help="skip the operator microbenchmark")
+    ap.add_argument("--lift-study", action="store_true",
+                    help="run only the lift conditioning study (single-face lift precision variants)")
     ap.add_argument("--stage-only", action="store_true",
                     help="run only a few fused mixed stages (for counter collection)")
     args = ap.parse_args()
@@ -143,6 +253,9 @@ def main():
     import bbdg
     from bbdg import basis, mesh
 
+    if args.lift_study:
+        print(json.dumps({"lift_conditioning": lift_study(bbdg.default_context())}))
+        return
     N, n = 9, args.n
     t0 = time.time()
     m = mesh.kuhn_mesh(n, 0.1, 3, periodic=False)
@@ -204,6 +317,12 @@ def main():
             traffic = rec["fetch_bytes"] + rec["write_bytes"]
     copy_gbs = stream_bandwidth(s.ctx)
     log(f"stream copy: {copy_gbs:.0f} GB/s")
+    ops = None
+    if not args.no_ops:
+        s.close()
+        ops = operator_bench(bbdg.default_context())
+        for k, v in ops.items():
+            log(f"operator {k}: {v['us']:.1f} us, {v['GBs']:.0f} GB/s ({v['frac_of_8TBs']:.2f} of 8 TB/s)")
     cpu = None
     if not args.no_cpu:
         thr = min(16, os.cpu_count() or 1)
@@ -242,6 +361,7 @@ def main():
                      "launch_us_graph": mx["ms_per_step"] * 1e3 / 5, "launch_us_eager": mx["stage_ms"] * 1e3,
                      "stream_copy_GBs": copy_gbs, "frac_of_stream_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
+        "operators_N7_K262144_F4": ops,
     }
     print(json.dumps(out))
 

@@ -72,6 +72,7 @@ def lib():
         L.orc_lsrk_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, dp, C.c_double, i32p, i8p, dp, dp, dp, dp,
                                     dp, dp, C.c_int]
         L.orc_lsrk_coeffs.argtypes = [dp, dp]
+        L.orc_face_lift_ld_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
         L.orc_flux.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, i32p, i8p, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
@@ -157,6 +158,15 @@ def face_lift(N, x):
     ncols = x.size // Nfp(N)
     out = np.zeros((ncols, Np(N)))
     lib().orc_face_lift_cols(N, ncols, x.reshape(-1), out.reshape(-1))
+    return out
+
+
+def face_lift_ld(N, x):
+    """face lift evaluated in long double, rounded to double at the end: (ncols, Nfp) -> (ncols, Np)"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    ncols = x.size // Nfp(N)
+    out = np.zeros((ncols, Np(N)))
+    lib().orc_face_lift_ld_cols(N, ncols, x.reshape(-1), out.reshape(-1))
     return out
 
 

@@ -630,3 +630,60 @@ void orc_flux(int pde, int flux_kind, int N, int64_t K, const double *uM, const 
     orc_mesh_t M = {N, F, pde, flux_kind, K, mapP, bc, nxJ, nyJ, nzJ, NULL, NULL, bc_sign};
     orc_interface_flux(&M, uM, flux);
 }
+
+/* ---------------------------------------------------------- extended precision */
+/* face_lift_multiply! (src/BernsteinLift.jl:139-164) carried out in long double (x87
+ * 64-bit mantissa): the reference against which the FP64 lift's own rounding is measured
+ * in the lift-conditioning study.  x, out are double; ncols columns. */
+void orc_face_lift_ld_cols(int N, int64_t ncols, const double *x, double *out) {
+    int64_t tri[24];
+    int nfp = orc_nfp(N), np = orc_np(N), nf1 = orc_nfp(N + 1);
+    long double *E = (long double *)malloc(sizeof(long double) * (nfp + nf1));
+    long double *Fs = E + nfp;
+    for (int64_t c = 0; c < ncols; ++c) {
+        const double *xc = x + c * nfp;
+        double *oc = out + c * np;
+        /* elevation N -> N+1 */
+        tri_ext(N, tri);
+        int row = 0;
+        for (int j = 0; j <= N + 1; ++j)
+            for (int i = 0; i <= N + 1 - j; ++i) {
+                int k = N + 1 - i - j;
+                long double v = 0.0L;
+                if (i > 0) v += (long double)i * xc[i - 1 + tri[j]];
+                if (j > 0) v += (long double)j * xc[i + tri[j - 1]];
+                if (k > 0) v += (long double)k * xc[i + tri[j]];
+                Fs[row++] = v / (N + 1);
+            }
+        /* reductions: degree n -> n-1 in place on a long double vector */
+        #define RED_LD(n, src, dst)                                                            \
+            do {                                                                               \
+                int64_t t_[24];                                                                \
+                tri_ext((n), t_);                                                              \
+                int r_ = 0;                                                                    \
+                for (int j_ = 0; j_ <= (n) - 1; ++j_)                                          \
+                    for (int i_ = 0; i_ <= (n) - 1 - j_; ++i_) {                               \
+                        long double v_ = (long double)(i_ + 1) * (src)[i_ + 1 + t_[j_]] +      \
+                                         (long double)(j_ + 1) * (src)[i_ + t_[j_ + 1]] +      \
+                                         (long double)((n) - i_ - j_) * (src)[i_ + t_[j_]];     \
+                        (dst)[r_++] = v_ / (n);                                                \
+                    }                                                                          \
+            } while (0)
+        RED_LD(N + 1, Fs, E);
+        long double scale = (long double)((N + 1) * (N + 1)) / 2.0L;
+        for (int m = 0; m < nfp; ++m) { E[m] *= scale; oc[m] = (double)E[m]; }
+        int index1 = nfp;
+        RED_LD(N, E, E);
+        for (int j = 1; j <= N; ++j) {
+            int diff = (N + 1 - j) * (N + 2 - j) / 2;
+            long double lj = ((j % 2) ? -1.0L : 1.0L) * (long double)binom(N, j) / (1.0L + j);
+            for (int i = 0; i < diff; ++i) oc[index1 + i] = (double)(lj * E[i]);
+            if (j < N) {
+                index1 += diff;
+                RED_LD(N - j, E, E);
+            }
+        }
+        #undef RED_LD
+    }
+    free(E);
+}

@@ -186,3 +186,31 @@ def test_lift_of_trace_of_constant(orc):
         assert np.isfinite(y).all()
         # L0 of a constant is (N+1)^2/2 * (E^T 1) = (N+1)^2/2 * (N+2)/(N+1) ... positive
         assert (orc.face_lift(N, np.ones((1, nfp(N))))[0, :nfp(N)] > 0).all()
+
+
+@pytest.mark.parametrize("N", [2, 5, 11, 17])
+def test_face_lift_long_double(orc, N):
+    """the extended-precision face lift agrees with the FP64 one to rounding, and (N=2) with
+    the exact rational naive stack"""
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal((8, nfp(N)))
+    ld = orc.face_lift_ld(N, x)
+    np.testing.assert_allclose(ld, orc.face_lift(N, x), rtol=0, atol=1e-13 * np.abs(ld).max())
+    if N == 2:
+        pts = [(Fr(a, N + 1), Fr(b, N + 1), Fr(c, N + 1)) for (a, b, c) in multi2(N + 1)]
+        E = solve([[psi(m, p) for m in multi2(N + 1)] for p in pts], [[psi(m, p) for m in multi2(N)] for p in pts])
+        Et = [list(r) for r in zip(*E)]
+        L0 = [[Fr((N + 1) ** 2, 2) * sum(Et[r][k] * E[k][c] for k in range(len(E))) for c in range(nfp(N))]
+              for r in range(nfp(N))]
+        xq = [Fr(v) for v in x[0]]
+        slab = [sum(L0[r][c] * xq[c] for c in range(nfp(N))) for r in range(nfp(N))]
+        exact = list(slab)
+        cur = slab
+        for j in range(1, N + 1):
+            d = N - j  # reduce degree d+1 -> d
+            pts = [(Fr(a, d + 1), Fr(b, d + 1), Fr(c, d + 1)) for (a, b, c) in multi2(d + 1)]
+            Ed = solve([[psi(m, p) for m in multi2(d + 1)] for p in pts], [[psi(m, p) for m in multi2(d)] for p in pts])
+            cur = [sum(Ed[k][r] * cur[k] for k in range(len(cur))) for r in range(nfp(d))]
+            lj = Fr((-1) ** j * comb(N, j), 1 + j)
+            exact += [lj * v for v in cur]
+        np.testing.assert_allclose(ld[0], [float(v) for v in exact], rtol=1e-15, atol=1e-15)
