@@ -130,6 +130,151 @@ __global__ __launch_bounds__(kLiftThreads) void k_lift4(const TI *__restrict__ x
     }
 }
 
+// Compact field-major layouts (x [K][F][4 Nfp], out [K][F][Np]): one block per element,
+// kLiftW threads.  The element's face data are copied into LDS with 16-byte
+// loads; L0 and every reduction step write their vectors into a per-(field, face) slab
+// array [F][4][Np] (slab s at offset tet(N, s), i.e. the reference's G1..G3 arrays before
+// the l_s weights); finally each lane gathers, for its volume nodes, the four face values
+// in the order G4 + G1 + G2 + G3 with the l_s weights in the accumulation type TA.  Work
+// items are (face, face node); the index and weight arithmetic of an item is shared by the
+// F fields, which are processed four at a time (reads of four fields, then their stores).
+constexpr int kLiftW = 128;  // measured: two waves per element beat one (64) and four (256)
+constexpr int kLiftFB = 4;
+
+template <int N, typename TI, typename TC, typename TA>
+__global__ __launch_bounds__(kLiftW) void k_lift4w(const TI *__restrict__ x, TI *__restrict__ out, int F, int64_t K) {
+    constexpr int NP = np3(N), NFP = nfp2(N);
+    extern __shared__ __align__(16) unsigned char lift_smem[];
+    TC *S = reinterpret_cast<TC *>(lift_smem);                                // [F][4][NP]
+    TI *X = reinterpret_cast<TI *>(lift_smem + (size_t)F * 4 * NP * sizeof(TC));  // [F][4][NFP]
+    __shared__ uint16_t dec[NP];
+    __shared__ double lw[N + 1];
+    const int tid = threadIdx.x;
+    const int64_t e = blockIdx.x;
+    // face data: F * 4 Nfp values, a multiple of 16 bytes for either type
+    constexpr int VW = 16 / sizeof(TI);
+    using V = std::conditional_t<sizeof(TI) == 8, double2, float4>;
+    const int nv = F * 4 * NFP / VW;
+    const V *xs = reinterpret_cast<const V *>(x + e * F * 4 * NFP);
+    V *xd = reinterpret_cast<V *>(X);
+    for (int t0 = 0; t0 < nv; t0 += 8 * kLiftW) {
+        V r[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int t = t0 + tid + u * kLiftW;
+            r[u] = xs[t < nv ? t : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int t = t0 + tid + u * kLiftW;
+            if (t < nv) xd[t] = r[u];
+        }
+    }
+    build_dec<N>(dec);
+    if (tid <= N) {
+#pragma unroll
+        for (int q = 0; q <= N; ++q)
+            if (tid == q) lw[q] = lj(N, q);
+    }
+    __syncthreads();
+    // L0 on every face of every field -> slab 0
+    for (int m = tid; m < 4 * NFP; m += kLiftW) {
+        const int phi = m / NFP, beta = m - phi * NFP;
+        const int ab = dec[np3(N - 1) + beta];
+        const L0Pt<TC> pt = l0_prep<N, TC>(ab & 255, ab >> 8);
+        for (int f0 = 0; f0 < F; f0 += kLiftFB) {
+            TC v[kLiftFB];
+#pragma unroll
+            for (int u = 0; u < kLiftFB; ++u) {
+                const int f = min(f0 + u, F - 1);
+                const TI *xf = X + (f * 4 + phi) * NFP;
+                TC s = TC(pt.w0) * TC(xf[pt.i0]);
+                s += TC(pt.w1) * TC(xf[pt.i1]);
+                s += TC(pt.w2) * TC(xf[pt.i2]);
+                s += TC(pt.w3) * TC(xf[pt.i3]);
+                s += TC(pt.w4) * TC(xf[pt.i4]);
+                s += TC(pt.w5) * TC(xf[pt.i5]);
+                s += TC(pt.w6) * TC(xf[pt.i6]);
+                v[u] = TC(0.5) * s;
+            }
+#pragma unroll
+            for (int u = 0; u < kLiftFB; ++u)
+                if (f0 + u < F) S[((f0 + u) * 4 + phi) * NP + beta] = v[u];
+        }
+    }
+    __syncthreads();
+    // the reduction chain: slab s (degree N-s) from slab s-1 (degree N-s+1)
+#pragma unroll
+    for (int sl = 1; sl <= N; ++sl) {
+        const int d = N - sl, nf = nfp2(d);
+        const int so = tet(N, sl - 1), dof = tet(N, sl);
+        const TC inv = TC(1) / TC(d + 1);
+        for (int m = tid; m < 4 * nf; m += kLiftW) {
+            const int phi = m / nf, beta = m - phi * nf;
+            const int ab = dec[np3(d - 1) + beta];
+            const int a = ab & 255, b = ab >> 8;
+            const int idx = so + lin2(d + 1, a, b);
+            const int o1 = idx + 1, o2 = idx + (d + 2 - b);
+            const TC wa = TC(a + 1), wb = TC(b + 1), wc = TC(d + 1 - a - b);
+            for (int f0 = 0; f0 < F; f0 += kLiftFB) {
+                TC v[kLiftFB];
+#pragma unroll
+                for (int u = 0; u < kLiftFB; ++u) {
+                    const int f = min(f0 + u, F - 1);
+                    const TC *sf = S + (f * 4 + phi) * NP;
+                    v[u] = fma(wa, sf[o1], fma(wb, sf[o2], wc * sf[idx])) * inv;
+                }
+#pragma unroll
+                for (int u = 0; u < kLiftFB; ++u)
+                    if (f0 + u < F) S[((f0 + u) * 4 + phi) * NP + dof + beta] = v[u];
+            }
+        }
+        __syncthreads();
+    }
+    // gather: out[n] = G4 + G1 + G2 + G3 with the l_s weights
+    TI *oe = out + e * F * NP;
+    for (int n = tid; n < NP; n += kLiftW) {
+        int i, j, k;
+        decode3<N>(n, i, j, k);
+        const int l = N - i - j - k;
+        // faces in the order f4, f1, f2, f3: slab and in-slab index of this node
+        const int s4 = k, s1 = j, s2 = l, s3 = i;
+        const int p4 = 3 * NP + tet(N, s4) + lin2(N - k, i, j);
+        const int p1 = 0 * NP + tet(N, s1) + lin2(N - j, i, k);
+        const int p2 = 1 * NP + tet(N, s2) + lin2(N - l, j, k);
+        const int p3 = 2 * NP + tet(N, s3) + lin2(N - i, j, k);
+        const TA w4 = TA(lw[s4]), w1 = TA(lw[s1]), w2 = TA(lw[s2]), w3 = TA(lw[s3]);
+        for (int f0 = 0; f0 < F; f0 += kLiftFB) {
+            TA v[kLiftFB];
+#pragma unroll
+            for (int u = 0; u < kLiftFB; ++u) {
+                const int f = min(f0 + u, F - 1);
+                const TC *sf = S + f * 4 * NP;
+                TA a = w4 * TA(sf[p4]);
+                a += w1 * TA(sf[p1]);
+                a += w2 * TA(sf[p2]);
+                a += w3 * TA(sf[p3]);
+                v[u] = a;
+            }
+#pragma unroll
+            for (int u = 0; u < kLiftFB; ++u)
+                if (f0 + u < F) oe[(f0 + u) * NP + n] = TI(v[u]);
+        }
+    }
+}
+
+template <int N, typename TI, typename TC, typename TA>
+bool try_lift4w(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
+    constexpr int NP = np3(N), NFP = nfp2(N);
+    if (lx.elem != (int64_t)F * 4 * NFP || lx.node != 1 || (F > 1 && lx.field != 4 * NFP)) return false;
+    if (lo.elem != (int64_t)F * NP || lo.node != 1 || (F > 1 && lo.field != NP)) return false;
+    if ((uintptr_t)x % 16) return false;
+    const size_t lds = (size_t)F * 4 * (NP * sizeof(TC) + NFP * sizeof(TI));
+    if (lds + NP * 2 + (N + 1) * 8 > 64 * 1024) return false;
+    k_lift4w<N, TI, TC, TA><<<(unsigned)K, kLiftW, lds, ctx->stream>>>((const TI *)x, (TI *)out, F, K);
+    return true;
+}
+
 // ---- float-float (compensated) arithmetic for the chain of the conditioning study
 struct FF {
     float hi, lo;
@@ -256,6 +401,10 @@ __global__ __launch_bounds__(kLiftThreads) void k_face_lift(const TI *__restrict
 
 template <int N, typename TI, typename TC, typename TA>
 int launch_lift4(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
+    if (K <= 0x7fffffff && try_lift4w<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo)) {
+        BB_LAUNCH_CHECK(ctx);
+        return BB_OK;
+    }
     const int64_t ncols = K * F;
     k_lift4<N, TI, TC, TA><<<grid1d(ncols, LiftCfg<N>::C), kLiftThreads, 0, ctx->stream>>>(
         (const TI *)x, lay(lx), (TI *)out, lay(lo), F, ncols);

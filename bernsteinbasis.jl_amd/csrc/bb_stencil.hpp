@@ -27,28 +27,29 @@ __host__ __device__ __forceinline__ void decode2(int n, int m, int &a, int &b) {
 // exactly the four weighted differences of fast_Dr_multiply! (src/BernsteinDerivativeMatrix_3D.jl:96-132),
 // fast_Ds_multiply! (:138-174) and fast_Dt_multiply! (:180-216), accumulated in the same
 // order.  A term whose weight is zero reads the centre value instead of an index that
-// does not exist (branch-free), and contributes 0 * finite = 0.
-template <int N, typename T>
-__device__ __forceinline__ void grad_ref(const T *x, int idx, int i, int j, int k, T &dr, T &ds, T &dt) {
+// does not exist (branch-free), and contributes 0 * finite = 0.  ld(m) returns the
+// coefficient of linear node m (a strided LDS read in the compact-layout kernels).
+template <int N, typename T, typename LD>
+__device__ __forceinline__ void grad_ref_ld(LD ld, int idx, int i, int j, int k, T &dr, T &ds, T &dt) {
     const int l = N - i - j - k;
     const int rl = N - k - j + 1;                    // length of row j in slab k
     const int sk = (N - k + 1) * (N - k + 2) / 2;    // size of slab k
     const int skm = (N - k + 2) * (N - k + 3) / 2;   // size of slab k-1
-    const T xc = x[idx];
-    const T xim = x[i > 0 ? idx - 1 : idx];          // (i-1,j,k)
-    const T xip = x[l > 0 ? idx + 1 : idx];          // (i+1,j,k)
-    const T xjm = x[j > 0 ? idx - rl - 1 : idx];     // (i,j-1,k)
-    const T xjmip = x[j > 0 ? idx - rl : idx];       // (i+1,j-1,k)
-    const T xjp = x[l > 0 ? idx + rl : idx];         // (i,j+1,k)
-    const T xjpim = x[i > 0 ? idx + rl - 1 : idx];   // (i-1,j+1,k)
-    const int km0 = idx - skm + j;                   // (i,j,k-1)
-    const T xkm = x[k > 0 ? km0 : idx];
-    const T xkmip = x[k > 0 ? km0 + 1 : idx];        // (i+1,j,k-1)
-    const T xkmjp = x[k > 0 ? km0 + rl + 1 : idx];   // (i,j+1,k-1)
-    const int kp0 = idx + sk - j;                    // (i,j,k+1)
-    const T xkp = x[l > 0 ? kp0 : idx];
-    const T xkpim = x[i > 0 ? kp0 - 1 : idx];        // (i-1,j,k+1)
-    const T xkpjm = x[j > 0 ? kp0 - rl : idx];       // (i,j-1,k+1)
+    const T xc = ld(idx);
+    const T xim = ld(i > 0 ? idx - 1 : idx);          // (i-1,j,k)
+    const T xip = ld(l > 0 ? idx + 1 : idx);          // (i+1,j,k)
+    const T xjm = ld(j > 0 ? idx - rl - 1 : idx);     // (i,j-1,k)
+    const T xjmip = ld(j > 0 ? idx - rl : idx);       // (i+1,j-1,k)
+    const T xjp = ld(l > 0 ? idx + rl : idx);         // (i,j+1,k)
+    const T xjpim = ld(i > 0 ? idx + rl - 1 : idx);   // (i-1,j+1,k)
+    const int km0 = idx - skm + j;                    // (i,j,k-1)
+    const T xkm = ld(k > 0 ? km0 : idx);
+    const T xkmip = ld(k > 0 ? km0 + 1 : idx);        // (i+1,j,k-1)
+    const T xkmjp = ld(k > 0 ? km0 + rl + 1 : idx);   // (i,j+1,k-1)
+    const int kp0 = idx + sk - j;                     // (i,j,k+1)
+    const T xkp = ld(l > 0 ? kp0 : idx);
+    const T xkpim = ld(i > 0 ? kp0 - 1 : idx);        // (i-1,j,k+1)
+    const T xkpjm = ld(j > 0 ? kp0 - rl : idx);       // (i,j-1,k+1)
     const T ti = T(i), tj = T(j), tk = T(k), tl = T(l), h = T(0.5);
     T v = T(0);
     v += ti * (xc - xim);
@@ -68,6 +69,10 @@ __device__ __forceinline__ void grad_ref(const T *x, int idx, int i, int j, int 
     v += tk * (xc - xkm);
     v += tl * (xkp - xc);
     dt = h * v;
+}
+template <int N, typename T>
+__device__ __forceinline__ void grad_ref(const T *x, int idx, int i, int j, int k, T &dr, T &ds, T &dt) {
+    grad_ref_ld<N, T>([x](int m) { return x[m]; }, idx, i, j, k, dr, ds, dt);
 }
 
 // L0 x at face node (a,b) of degree N:  L0 = (N+1)^2/2 (E_N^{N+1})^T E_N^{N+1}

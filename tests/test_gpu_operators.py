@@ -105,6 +105,45 @@ def test_grad(ctx, orc, N, dtype, tol):
         assert rel(from_dev_cols(outs[c][1]), ref) <= tol, (N, c, rel(from_dev_cols(outs[c][1]), ref))
 
 
+@pytest.mark.parametrize("N", [1, 4, 7, 8])
+@pytest.mark.parametrize("F", [1, 3, 4])
+@pytest.mark.parametrize("layout", ["aos", "soa", "padded"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, TOL64), (np.float32, 2e-6)])
+def test_grad_many_blocks(ctx, orc, N, F, layout, dtype, tol):
+    """enough elements for many blocks and a partial last block, odd node counts (scalar
+    loads), both compact layouts, and a padded element stride (the strided kernel)"""
+    import bbdg
+
+    rng = np.random.default_rng(N * 10 + F)
+    K, n = 1003, np3(N)
+    x = rng.uniform(-1, 1, (K, F, n))
+    geo_h = rng.uniform(-2, 2, (K, 9))
+    geo = bbdg.DeviceArray.from_numpy(geo_h)
+    if layout == "padded":
+        pad = np.zeros((K, F, n + 3), dtype)
+        pad[:, :, :n] = x
+        xd = bbdg.DeviceArray.from_numpy(pad)
+        xv = xd.view((n, K, F), (1, F * (n + 3), n + 3))
+        outs = []
+        for _ in range(3):
+            d = bbdg.DeviceArray.zeros((K, F, n + 3), dtype)
+            outs.append((d, d.view(xv.shape, xv.strides)))
+    else:
+        _, xv = to_dev_cols(x, layout, dtype)
+        outs = [empty_cols(n, K, F, layout, dtype) for _ in range(3)]
+    bbdg.grad_(outs[0][1], outs[1][1], outs[2][1], N, xv, geo)
+    xr = x.astype(dtype).astype(np.float64).reshape(-1, n)
+    d = [orc.deriv_mul(a, N, xr).reshape(K, F, -1) for a in range(3)]
+    g = geo_h[:, :, None, None]
+    for c in range(3):
+        ref = g[:, 3 * c + 0] * d[0] + g[:, 3 * c + 1] * d[1] + g[:, 3 * c + 2] * d[2]
+        got = from_dev_cols(outs[c][1])
+        assert rel(got, ref) <= tol, (N, F, layout, c, rel(got, ref))
+    D = bbdg.BernsteinDerivativeMatrix_3D_s(N)
+    bbdg.mul_(outs[0][1], D, xv)
+    assert rel(from_dev_cols(outs[0][1]), d[1]) <= tol
+
+
 @pytest.mark.parametrize("N", list(range(1, 13)) + [15, 17, 20])
 def test_lift_fp64(ctx, orc, N):
     import bbdg
@@ -118,6 +157,30 @@ def test_lift_fp64(ctx, orc, N):
         bbdg.mul_(ov, bbdg.BernsteinLift(N), xv)
         ref = orc.lift(N, x.reshape(-1, 4 * nfp(N))).reshape(K, F, -1)
         assert rel(from_dev_cols(ov), ref) <= TOL64, (N, layout, rel(from_dev_cols(ov), ref))
+
+
+@pytest.mark.parametrize("N", [1, 3, 7, 9, 12])
+@pytest.mark.parametrize("F", [1, 4, 5, 6])
+def test_lift_field_counts(ctx, orc, N, F):
+    """field counts around the four-field batches of the one-wave-per-element kernel (F=5
+    leaves a remainder of one), in both compact layouts and all precision modes"""
+    import bbdg
+
+    rng = np.random.default_rng(500 + 10 * N + F)
+    K = 37
+    x = rng.uniform(-1, 1, (K, F, 4 * nfp(N)))
+    ref = orc.lift(N, x.reshape(-1, 4 * nfp(N))).reshape(K, F, -1)
+    ref32 = orc.lift(N, x.astype(np.float32).astype(np.float64).reshape(-1, 4 * nfp(N))).reshape(K, F, -1)
+    for layout in ("soa", "aos"):
+        _, xv = to_dev_cols(x, layout, np.float64)
+        _, ov = empty_cols(np3(N), K, F, layout, np.float64)
+        bbdg.mul_(ov, bbdg.BernsteinLift(N), xv)
+        assert rel(from_dev_cols(ov), ref) <= TOL64, (layout, rel(from_dev_cols(ov), ref))
+        for mode in ("plain", "acc64"):
+            _, xv = to_dev_cols(x, layout, np.float32)
+            _, ov = empty_cols(np3(N), K, F, layout, np.float32)
+            bbdg.mul_(ov, bbdg.BernsteinLift(N, np.float32, mode), xv)
+            assert rel(from_dev_cols(ov), ref32) <= 1e-5, (layout, mode, rel(from_dev_cols(ov), ref32))
 
 
 @pytest.mark.parametrize("N", [2, 5, 7, 9])
@@ -158,6 +221,28 @@ def test_face_lift(ctx, orc, N):
         errs[mode] = rel(from_dev_cols(ov), ref32)
     assert errs["plain"] <= 1e-5
     assert errs["acc64"] <= 1e-5 and errs["twosum"] <= 1e-5
+
+
+@pytest.mark.parametrize("N", [8, 11, 14, 17])
+def test_face_lift_fp64_conditioning(ctx, orc, N):
+    """the FP64 single-face lift against the long-double restatement, on the inputs of the
+    lift conditioning study: N(0,1) face coefficients (seed 4) and the same scaled per
+    column by 10^U[-3,3]; the relative L2 error stays at FP64 rounding level"""
+    import bbdg
+
+    rng = np.random.default_rng(4)
+    K, nf, n = 512, nfp(N), np3(N)
+    base = rng.standard_normal((K, nf))
+    scale = 10.0 ** rng.uniform(-3, 3, (K, 1))
+    L = bbdg.BernsteinLift(N)
+    for x in (base, base * scale):
+        xd = bbdg.DeviceArray.from_numpy(x)
+        od = bbdg.DeviceArray.zeros((K, n))
+        bbdg.face_lift_multiply_(od.view((n, K), (1, n)), L, xd.view((nf, K), (1, nf)))
+        ld = orc.face_lift_ld(N, x)
+        err = rel(od.numpy(), ld)
+        print(f"N={N} fp64 vs long double: {err:.3e}")
+        assert err <= 1e-14, (N, err)
 
 
 @pytest.mark.parametrize("N", [1, 2, 4, 7, 10, 20])
