@@ -27,6 +27,7 @@ struct bb_solver {
     double *geo = nullptr, *J = nullptr, *nJ = nullptr;  // nJ: [K][4][5] nxJ nyJ nzJ Jf 1/Jf
     int2 *conn = nullptr;
     uint16_t *tab = nullptr;
+    float *l0a = nullptr;  // L0 as MFMA A operands (mixed precision, matrix-core L0 tilings)
     std::vector<int2> conn_host;
     std::vector<double> nJ_host;
     // plain-path mesh arrays (StartUpDG layout) and scratch, allocated on first use
@@ -107,6 +108,7 @@ static StageArgs stage_args(bb_solver *s, int prec) {
     a.nJ = s->nJ;
     a.conn = s->conn;
     a.tab = s->tab;
+    a.l0a = s->l0a;
     a.K = s->K;
     a.flux_kind = s->flux_kind;
     for (int f = 0; f < 6; ++f) a.sign[f] = s->sign[f];
@@ -295,16 +297,19 @@ int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, cons
         o[4] = 1.0 / Jf;
     }
     const std::vector<uint16_t> tab = stage_tables(N);
+    const std::vector<float> l0a = l0_mfma_table(N);
     if ((e = hipMalloc(&s->geo, K * 9 * sizeof(double))) || (e = hipMalloc(&s->J, K * sizeof(double))) ||
         (e = hipMalloc(&s->nJ, K * 20 * sizeof(double))) || (e = hipMalloc(&s->conn, K * 4 * sizeof(int2))) ||
         (e = hipMalloc(&s->tab, tab.size() * sizeof(uint16_t))) ||
+        (e = hipMalloc(&s->l0a, l0a.size() * sizeof(float))) ||
         (e = hipMalloc(&s->q[0], vw * sizeof(double))) || (e = hipMalloc(&s->q[1], vw * sizeof(double))))
         return fail(e);
     if ((e = hipMemcpy(s->geo, geo_host, K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->J, J_host, K * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->nJ, nj5.data(), K * 20 * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->conn, conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->tab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice)))
+        (e = hipMemcpy(s->tab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->l0a, l0a.data(), l0a.size() * sizeof(float), hipMemcpyHostToDevice)))
         return fail(e);
     if ((rc = bb_fill_zero(ctx, s->q[0], vw * sizeof(double))) || (rc = bb_fill_zero(ctx, s->q[1], vw * sizeof(double)))) {
         bb_solver_destroy(s);
@@ -318,7 +323,7 @@ int bb_solver_destroy(bb_solver *s) {
     if (!s) return BB_ERR_NULL;
     hipStreamSynchronize(s->ctx->stream);
     drop_graphs(s);
-    void *ptrs[] = {s->geo, s->J, s->nJ, s->conn, s->tab, s->mapP, s->bcn, s->nx, s->ny, s->nz, s->uM, s->flux, s->du,
+    void *ptrs[] = {s->geo, s->J, s->nJ, s->conn, s->tab, s->l0a, s->mapP, s->bcn, s->nx, s->ny, s->nz, s->uM, s->flux, s->du,
                     s->q[0], s->q[1], s->res64, s->res32};
     for (void *p : ptrs)
         if (p) hipFree(p);

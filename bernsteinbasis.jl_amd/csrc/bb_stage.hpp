@@ -35,6 +35,7 @@ struct StageArgs {
     const double *nJ;      // [K][4][5]: nxJ nyJ nzJ Jf 1/Jf per face
     const int2 *conn;
     const uint16_t *tab;   // index tables of degree N (stage_tables)
+    const float *l0a;      // L0 in the A-operand layout of the FP32 MFMA (l0_mfma_table)
     int64_t K;
     double ca, cb, dt;
     int first;     // stage 1: res is not read (a_1 = 0)
@@ -51,7 +52,11 @@ struct StageArgs {
 //  PINGPONG : only the current and next chain vectors are kept; after every step each
 //             thread adds the values that land on its node.  LDS: ~8 F Nfp values, but the
 //             per-step accumulation diverges across the lanes of a wave.
-enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1 };
+//  GATHER_MFMA (FP32 only): GATHER, with slab 0 = L0 x of all the block's chains computed
+//             as one product on the matrix cores: L0 (Nfp x Nfp, zero-padded to 16-row
+//             tiles and 4-deep k steps) times the chain columns (element, field, face) in
+//             v_mfma_f32_16x16x4f32 tiles with FP32 accumulation.
+enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1, LIFT_GATHER_MFMA = 2 };
 
 template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct StageCfg {
     using T = std::conditional_t<MIXED, float, double>;
@@ -60,14 +65,43 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct Stage
     static constexpr int NT = ((EB * NP + 63) / 64) * 64;
     // PINGPONG: s1 = state, then chain A; s2 = flux, then chain B
     // GATHER:   s1 = state, then all chain vectors (EB F 4 Np); s2 = flux
-    static constexpr int R1 = LM == LIFT_GATHER ? EB * F * 4 * NP
+    static constexpr bool GATHER = LM == LIFT_GATHER || LM == LIFT_GATHER_MFMA;
+    static constexpr int R1 = GATHER ? EB * F * 4 * NP
                               : ((EB * F * NP > EB * F * 4 * NFP) ? EB * F * NP : EB * F * 4 * NFP);
     static constexpr int R2 = EB * F * 4 * NFP;
     static constexpr int TAB = 2 * NP + 28 * NFP;  // ijk, dec, own face nodes, neighbour face nodes
     static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
     static constexpr size_t lds = (size_t)(R1 + R2) * sizeof(T) + (size_t)TAB16 * 16 + (N + 1) * 8;
-    static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024;
+    // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
+    static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
+    static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024 && (LM != LIFT_GATHER_MFMA || MIXED);
 };
+
+// L0 = (N+1)^2/2 (E_N^{N+1})^T E_N^{N+1} as a dense Nfp x Nfp matrix (the 7-point stencil
+// of l0_apply; every entry is a multiple of 1/2, exact in FP32), laid out as the A operand
+// of v_mfma_f32_16x16x4f32: for row tile mt, lane L and k step kk the value
+// L0[16 mt + L % 16][4 kk + L / 16] (zero outside the matrix), stored [mt][L][kk].
+inline std::vector<float> l0_mfma_table(int N) {
+    const int NFP = nfp2(N), MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
+    std::vector<double> L0((size_t)NFP * NFP, 0.0);
+    for (int b = 0; b <= N; ++b)
+        for (int a = 0; a <= N - b; ++a) {
+            const int c = N - a - b, r = lin2(N, a, b);
+            const int nb[7][3] = {{a, b, (a + 1) * (a + 1) + (b + 1) * (b + 1) + (c + 1) * (c + 1)},
+                                  {a + 1, b - 1, (a + 1) * b}, {a + 1, b, (a + 1) * c}, {a - 1, b + 1, (b + 1) * a},
+                                  {a, b + 1, (b + 1) * c},     {a - 1, b, (c + 1) * a}, {a, b - 1, (c + 1) * b}};
+            for (auto &q : nb)
+                if (q[2] != 0) L0[(size_t)r * NFP + lin2(N, q[0], q[1])] += 0.5 * q[2];
+        }
+    std::vector<float> t((size_t)MT * 64 * KT, 0.f);
+    for (int mt = 0; mt < MT; ++mt)
+        for (int L = 0; L < 64; ++L)
+            for (int kk = 0; kk < KT; ++kk) {
+                const int row = 16 * mt + L % 16, col = 4 * kk + L / 16;
+                if (row < NFP && col < NFP) t[((size_t)mt * 64 + L) * KT + kk] = (float)L0[(size_t)row * NFP + col];
+            }
+    return t;
+}
 
 // Index tables of degree N for the stage kernel, built once on the host (uint16 each):
 //   [0, Np)            node n -> i | j << 4 | k << 8
@@ -234,8 +268,45 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     // volume offset tet(N,s)); PINGPONG keeps two [chain][Nfp] buffers.  Work items are
     // (element, face, face node); the index and weight arithmetic is done once per item and
     // reused for the F fields of the item.
-    constexpr int QS = LM == LIFT_GATHER ? NP : NFP;
+    constexpr int QS = Cfg::GATHER ? NP : NFP;
     // ---- L0 on every (element, face): flux s2 -> slab 0 (buffer s1)
+    if constexpr (LM == LIFT_GATHER_MFMA) {
+        // chain column col = (element, field, face) is s2[col * Nfp ..] and s1[col * Np ..];
+        // work unit (mt, nt) = row tile x 16 columns, dealt round-robin to the waves
+        constexpr int MT = Cfg::MT, KT = Cfg::KT, NW = NT / 64;
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        const int ncols = ne * F * 4;
+        const int nunits = MT * ((ncols + 15) / 16);
+        const int wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid & 63;
+        int mt_loaded = -1;
+        float a[KT];
+        for (int u = wave; u < nunits; u += NW) {
+            const int mt = u % MT, nt = u / MT;
+            if (mt != mt_loaded) {
+                const float *ap = A.l0a + ((size_t)mt * 64 + lane) * KT;
+#pragma unroll
+                for (int kk = 0; kk < KT; ++kk) a[kk] = ap[kk];
+                mt_loaded = mt;
+            }
+            const int col = nt * 16 + (lane & 15);
+            const T *xb = s2 + (col < ncols ? col : ncols - 1) * NFP;
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < KT; ++kk) {
+                const int k = 4 * kk + (lane >> 4);
+                // rows of the padded k range meet zero columns of L0, but must be finite
+                const float b = (4 * kk + 3 < NFP || k < NFP) ? xb[k < NFP ? k : 0] : 0.f;
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kk], b, acc, 0, 0, 0);
+            }
+            if (col < ncols) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int beta = 16 * mt + 4 * (lane >> 4) + r;
+                    if (beta < NFP) s1[col * QS + beta] = acc[r];
+                }
+            }
+        }
+    } else
     for (int t = tid; t < ne * 4 * NFP; t += NT) {
         const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, beta = t % NFP;
         const int ab = dec[np3(N - 1) + beta];
@@ -275,7 +346,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
         }
     };
 
-    if constexpr (LM == LIFT_GATHER) {
+    if constexpr (Cfg::GATHER) {
 #pragma unroll
         for (int s = 1; s <= N; ++s) {
             chain_step(s1, s1, tet(N, s - 1), tet(N, s), N - s);
@@ -347,7 +418,7 @@ int stage_launch_maxwell_mixed(int N, int tiling, const StageArgs &a, hipStream_
 constexpr int kStageMaxN = 10;
 // candidate tilings (elements per block, lift organisation), in autotuning order
 constexpr int kStageEB[] = {1, 2, 4};
-constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG};
+constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG, LIFT_GATHER_MFMA};
 
 // tiling code: elements per block + 16 * lift organisation
 constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
@@ -377,6 +448,10 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             };                                                                                         \
             using G0 = std::integral_constant<int, LIFT_GATHER>;                                       \
             using P1 = std::integral_constant<int, LIFT_PINGPONG>;                                     \
+            using M2 = std::integral_constant<int, LIFT_GATHER_MFMA>;                                  \
+            one(std::integral_constant<int, 1>{}, M2{});                                               \
+            one(std::integral_constant<int, 2>{}, M2{});                                               \
+            one(std::integral_constant<int, 4>{}, M2{});                                               \
             one(std::integral_constant<int, 1>{}, G0{});                                               \
             one(std::integral_constant<int, 2>{}, G0{});                                               \
             one(std::integral_constant<int, 4>{}, G0{});                                               \

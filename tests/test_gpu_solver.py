@@ -98,11 +98,49 @@ def test_autotune_and_tiling_invariance(ctx):
         assert np.array_equal(outs[(lift, 4)], outs[(lift, 1)])
     t = s.autotune("f64")
     assert t in [s.tiling_code(e, lm) for e in (1, 2, 4) for lm in ("gather", "pingpong")]
+    tm = s.autotune("mixed")
+    assert tm in [s.tiling_code(e, lm) for e in (1, 2, 4) for lm in ("gather", "pingpong", "gather_mfma")]
     assert rel(s.rhs("fused", "f64"), ref) <= TOL64
     for prec in ("f64", "mixed"):
         s.set_tuning(prec, s.tiling_code(1, "pingpong"))
     for path, prec, tol in (("fused", "f64", TOL64), ("fused", "mixed", 2e-5)):
         assert rel(s.rhs(path, prec), ref) <= tol
+
+
+@pytest.mark.parametrize("pde,N,n", [("wave", 1, 2), ("wave", 4, 3), ("wave", 7, 2), ("wave", 10, 2),
+                                     ("maxwell", 3, 3), ("maxwell", 6, 2), ("maxwell", 9, 2)])
+def test_mfma_l0_tilings(ctx, pde, N, n):
+    """L0 on the matrix cores (mixed precision): every compiled elements-per-workgroup
+    count, partial last workgroups (K = 6 n^3 is not a multiple of 4), against the oracle
+    and against the same tiling with L0 on the vector units, for the RHS and one step"""
+    import bbdg
+
+    m, q0, s, ma, O = setup(pde, N, n=n)
+    p = 0 if pde == "wave" else 1
+    ref = O.rhs(p, N, q0, ma, nthreads=4, flux_kind=1)
+    dt = 0.5 * m.h / N ** 2
+    ran = 0
+    for eb in (1, 2, 4):
+        try:
+            s.set_tuning("mixed", s.tiling_code(eb, "gather_mfma"))
+        except bbdg.BBError:
+            continue
+        ran += 1
+        got = s.rhs("fused", "mixed")
+        assert rel(got, ref) <= 2e-5, (eb, rel(got, ref))
+        s.set_tuning("mixed", s.tiling_code(eb, "gather"))
+        valu = s.rhs("fused", "mixed")
+        assert rel(got, valu) <= 2e-6, (eb, rel(got, valu))
+        s.set_tuning("mixed", s.tiling_code(eb, "gather_mfma"))
+        s.set_state(q0)
+        s.step(dt, 1, "fused", "mixed")
+        a = s.get_state()
+        s.set_tuning("mixed", s.tiling_code(eb, "gather"))
+        s.set_state(q0)
+        s.step(dt, 1, "fused", "mixed")
+        assert rel(a, s.get_state()) <= 1e-6
+        s.set_state(q0)
+    assert ran >= 1
 
 
 def test_config1_every_step(ctx):
