@@ -298,6 +298,10 @@ int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, cons
     }
     const std::vector<uint16_t> tab = stage_tables(N);
     const std::vector<float> l0a = l0_mfma_table(N);
+    if (l0a.empty()) {
+        delete s;
+        return bb_fail(ctx, BB_ERR_STATE, "L0 has entries outside the band the matrix-core kernel assumes (N=%d)", N);
+    }
     if ((e = hipMalloc(&s->geo, K * 9 * sizeof(double))) || (e = hipMalloc(&s->J, K * sizeof(double))) ||
         (e = hipMalloc(&s->nJ, K * 20 * sizeof(double))) || (e = hipMalloc(&s->conn, K * 4 * sizeof(int2))) ||
         (e = hipMalloc(&s->tab, tab.size() * sizeof(uint16_t))) ||

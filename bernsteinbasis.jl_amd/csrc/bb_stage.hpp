@@ -77,6 +77,14 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct Stage
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024 && (LM != LIFT_GATHER_MFMA || MIXED);
 };
 
+// L0 couples face nodes at most N+1 apart in linear order (the 7-point stencil reaches the
+// neighbouring rows, whose offsets are N+1-b and N+2-b for row b >= 1): the k steps of row
+// tile mt that can hold non-zero entries
+__host__ __device__ constexpr int l0_klo(int N, int mt) { return (16 * mt - (N + 1)) > 0 ? (16 * mt - (N + 1)) / 4 : 0; }
+__host__ __device__ constexpr int l0_khi(int N, int mt) {
+    return ((16 * mt + 15 + (N + 1)) < nfp2(N) - 1 ? (16 * mt + 15 + (N + 1)) : nfp2(N) - 1) / 4;
+}
+
 // L0 = (N+1)^2/2 (E_N^{N+1})^T E_N^{N+1} as a dense Nfp x Nfp matrix (the 7-point stencil
 // of l0_apply; every entry is a multiple of 1/2, exact in FP32), laid out as the A operand
 // of v_mfma_f32_16x16x4f32: for row tile mt, lane L and k step kk the value
@@ -94,12 +102,16 @@ inline std::vector<float> l0_mfma_table(int N) {
                 if (q[2] != 0) L0[(size_t)r * NFP + lin2(N, q[0], q[1])] += 0.5 * q[2];
         }
     std::vector<float> t((size_t)MT * 64 * KT, 0.f);
+    bool outside = false;
     for (int mt = 0; mt < MT; ++mt)
         for (int L = 0; L < 64; ++L)
             for (int kk = 0; kk < KT; ++kk) {
                 const int row = 16 * mt + L % 16, col = 4 * kk + L / 16;
                 if (row < NFP && col < NFP) t[((size_t)mt * 64 + L) * KT + kk] = (float)L0[(size_t)row * NFP + col];
+                if (t[((size_t)mt * 64 + L) * KT + kk] != 0.f && (kk < l0_klo(N, mt) || kk > l0_khi(N, mt)))
+                    outside = true;
             }
+    if (outside) t.clear();  // an entry the kernel would skip: refuse the table
     return t;
 }
 
@@ -172,6 +184,18 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
 #pragma unroll
         for (int s = 0; s <= N; ++s)
             if (tid == s) lw[s] = lj(N, s);
+    }
+    // matrix-core L0: this wave's first row tile of L0 as A operands, needed after the flux
+    constexpr int MT = Cfg::MT, KT = Cfg::KT, NW = NT / 64;
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const int wave_u = __builtin_amdgcn_readfirstlane(tid / 64);
+    float amf[LM == LIFT_GATHER_MFMA ? KT : 1];
+    if constexpr (LM == LIFT_GATHER_MFMA) {
+        if (wave_u < MT) {
+            const float *ap = A.l0a + ((size_t)wave_u * 64 + (tid & 63)) * KT;
+#pragma unroll
+            for (int kk = 0; kk < KT; ++kk) amf[kk] = ap[kk];
+        }
     }
     using TR = T;
     TR *res = reinterpret_cast<TR *>(A.res);
@@ -271,38 +295,52 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
     constexpr int QS = Cfg::GATHER ? NP : NFP;
     // ---- L0 on every (element, face): flux s2 -> slab 0 (buffer s1)
     if constexpr (LM == LIFT_GATHER_MFMA) {
-        // chain column col = (element, field, face) is s2[col * Nfp ..] and s1[col * Np ..];
-        // work unit (mt, nt) = row tile x 16 columns, dealt round-robin to the waves
-        constexpr int MT = Cfg::MT, KT = Cfg::KT, NW = NT / 64;
-        typedef float v4f __attribute__((ext_vector_type(4)));
+        // chain column col = (element, field, face) is s2[col * Nfp ..] and s1[col * Np ..].
+        // Wave w computes the row tiles mt = w, w + NW, .. for all column tiles at once:
+        // the B operands of every column tile are read first, then the k steps advance
+        // the column tiles' independent accumulators in turn.
+        constexpr int NTLM = (EB * F * 4 + 15) / 16;
         const int ncols = ne * F * 4;
-        const int nunits = MT * ((ncols + 15) / 16);
-        const int wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid & 63;
-        int mt_loaded = -1;
-        float a[KT];
-        for (int u = wave; u < nunits; u += NW) {
-            const int mt = u % MT, nt = u / MT;
-            if (mt != mt_loaded) {
+        const int lane = tid & 63;
+        for (int mt = wave_u; mt < MT; mt += NW) {
+            if (mt != wave_u) {
                 const float *ap = A.l0a + ((size_t)mt * 64 + lane) * KT;
 #pragma unroll
-                for (int kk = 0; kk < KT; ++kk) a[kk] = ap[kk];
-                mt_loaded = mt;
+                for (int kk = 0; kk < KT; ++kk) amf[kk] = ap[kk];
             }
-            const int col = nt * 16 + (lane & 15);
-            const T *xb = s2 + (col < ncols ? col : ncols - 1) * NFP;
-            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            const int klo = l0_klo(N, mt), khi = l0_khi(N, mt);
+            float b[NTLM][KT];
+#pragma unroll
+            for (int nt = 0; nt < NTLM; ++nt) {
+                const int col = nt * 16 + (lane & 15);
+                const T *xb = s2 + (col < ncols ? col : ncols - 1) * NFP;
+#pragma unroll
+                for (int kk = 0; kk < KT; ++kk) {
+                    if (kk < klo || kk > khi) continue;
+                    const int k = 4 * kk + (lane >> 4);
+                    // rows of the padded k range meet zero columns of L0, but must be finite
+                    b[nt][kk] = (4 * kk + 3 < NFP || k < NFP) ? xb[k < NFP ? k : NFP - 1] : 0.f;
+                }
+            }
+            v4f acc[NTLM];
+#pragma unroll
+            for (int nt = 0; nt < NTLM; ++nt) acc[nt] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kk = 0; kk < KT; ++kk) {
-                const int k = 4 * kk + (lane >> 4);
-                // rows of the padded k range meet zero columns of L0, but must be finite
-                const float b = (4 * kk + 3 < NFP || k < NFP) ? xb[k < NFP ? k : 0] : 0.f;
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kk], b, acc, 0, 0, 0);
-            }
-            if (col < ncols) {
+                if (kk < klo || kk > khi) continue;  // an all-zero tile of L0
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int beta = 16 * mt + 4 * (lane >> 4) + r;
-                    if (beta < NFP) s1[col * QS + beta] = acc[r];
+                for (int nt = 0; nt < NTLM; ++nt)
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(amf[kk], b[nt][kk], acc[nt], 0, 0, 0);
+            }
+#pragma unroll
+            for (int nt = 0; nt < NTLM; ++nt) {
+                const int col = nt * 16 + (lane & 15);
+                if (col < ncols) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int beta = 16 * mt + 4 * (lane >> 4) + r;
+                        if (beta < NFP) s1[col * QS + beta] = acc[nt][r];
+                    }
                 }
             }
         }
