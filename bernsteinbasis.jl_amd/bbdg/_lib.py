@@ -78,6 +78,17 @@ SIGS = {
     "bb_solver_autotune": (i32, [vp, i32, C.POINTER(C.c_int)]),
     "bb_solver_set_tuning": (i32, [vp, i32, i32]),
     "bb_solver_stage_bytes": (i32, [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bb_batch_create": (i32, [vp, i32, i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64, C.POINTER(vp)]),
+    "bb_batch_destroy": (i32, [vp]),
+    "bb_batch_state_words": (i32, [vp, C.POINTER(C.c_int64)]),
+    "bb_batch_set_state": (i32, [vp, vp]),
+    "bb_batch_get_state": (i32, [vp, vp]),
+    "bb_batch_rhs": (i32, [vp, i32, vp]),
+    "bb_batch_step": (i32, [vp, i32, dbl, i32]),
+    "bb_batch_graph_capture": (i32, [vp, i32, dbl, i32]),
+    "bb_batch_graph_launch": (i32, [vp]),
+    "bb_batch_autotune": (i32, [vp, i32, C.POINTER(C.c_int)]),
+    "bb_batch_set_tuning": (i32, [vp, i32, i32, i32]),
 }
 
 _lib = None

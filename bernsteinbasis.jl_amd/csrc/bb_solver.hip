@@ -11,13 +11,6 @@
 
 using namespace bb;
 
-// Five-stage fourth-order 2N-storage RK (Carpenter & Kennedy, NASA TM-109112, 1994)
-static const double kA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
-                             -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
-static const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
-                             1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
-                             2277821191437.0 / 14882151754819.0};
-
 struct bb_solver {
     bb_ctx *ctx = nullptr;
     int pde = 0, F = 0, N = 0, flux_kind = 1, NP = 0, NFP = 0;
@@ -197,14 +190,14 @@ static int enqueue_steps(bb_solver *s, int path, int prec, double dt, int nsteps
         for (int stage = 0; stage < 5; ++stage) {
             if (path == BB_PATH_PLAIN) {
                 if ((rc = plain_rhs(s, s->q[*cur]))) return rc;
-                if ((rc = bb_lsrk_update(s->ctx, nw, s->q[*cur], s->res64, s->du, kA[stage], kB[stage], dt, stage == 0)))
+                if ((rc = bb_lsrk_update(s->ctx, nw, s->q[*cur], s->res64, s->du, kLsrkA[stage], kLsrkB[stage], dt, stage == 0)))
                     return rc;
             } else {
                 StageArgs a = stage_args(s, prec);
                 a.q_in = s->q[*cur];
                 a.q_out = s->q[*cur ^ 1];
-                a.ca = kA[stage];
-                a.cb = kB[stage];
+                a.ca = kLsrkA[stage];
+                a.cb = kLsrkB[stage];
                 a.dt = dt;
                 a.first = stage == 0;
                 if ((rc = launch_stage(s, prec, s->tiling[prec], a))) return rc;
@@ -459,8 +452,8 @@ int bb_solver_autotune(bb_solver *s, int prec, int *tiling) {
     StageArgs a = stage_args(s, prec);
     a.q_in = s->q[s->cur];
     a.q_out = s->q[s->cur ^ 1];  // the current state is only read
-    a.ca = kA[1];
-    a.cb = kB[1];
+    a.ca = kLsrkA[1];
+    a.cb = kLsrkB[1];
     a.dt = 0.0;
     a.first = 0;
     float best = 1e30f;

@@ -42,7 +42,25 @@ struct StageArgs {
     int rhs_only;  // write rhs into res, leave q alone
     int flux_kind;
     double sign[6];
+    // elements of mixed degree (k_stage with MD = true): the neighbour across face f of
+    // element e has its state at q_nb + nboff[4 e + f] and its degree in bits 8..11 of
+    // conn.y; traces of another degree are brought to this degree by the matrices
+    // tr + troff[11 N + Nn] (nfp(N) x nfp(Nn), row-major); element e steps by dt * dte[e]
+    const double *q_nb;
+    const int64_t *nboff;
+    const double *tr;
+    const int64_t *troff;
+    const double *dte;
 };
+
+// Five-stage fourth-order 2N-storage RK (Carpenter & Kennedy, NASA TM-109112, 1994)
+constexpr double kLsrkA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                              -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+constexpr double kLsrkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                              1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                              2277821191437.0 / 14882151754819.0};
+
+constexpr int kTraceMax = 66;  // face nodes of the highest degree of a fused kernel (N = 10)
 
 // Lift organisation inside the stage kernel:
 //  GATHER   : every chain vector of every face is kept in LDS (slab s of face f at the
@@ -58,7 +76,7 @@ struct StageArgs {
 //             v_mfma_f32_16x16x4f32 tiles with FP32 accumulation.
 enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1, LIFT_GATHER_MFMA = 2 };
 
-template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct StageCfg {
+template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = false> struct StageCfg {
     using T = std::conditional_t<MIXED, float, double>;
     static constexpr int F = Pde<PDE>::F;
     static constexpr int NP = np3(N), NFP = nfp2(N);
@@ -71,7 +89,8 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER> struct Stage
     static constexpr int R2 = EB * F * 4 * NFP;
     static constexpr int TAB = 2 * NP + 28 * NFP;  // ijk, dec, own face nodes, neighbour face nodes
     static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
-    static constexpr size_t lds = (size_t)(R1 + R2) * sizeof(T) + (size_t)TAB16 * 16 + (N + 1) * 8;
+    static constexpr int R3 = MD ? EB * 4 * F * kTraceMax : 0;  // neighbour traces of other degrees
+    static constexpr size_t lds = (size_t)(R1 + R2 + R3) * sizeof(T) + (size_t)TAB16 * 16 + (N + 1) * 8;
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
     static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024 && (LM != LIFT_GATHER_MFMA || MIXED);
@@ -150,14 +169,15 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
     return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
 }
 
-template <int N, int PDE, bool MIXED, int EB, int LM>
-__global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage(StageArgs A) {
-    using Cfg = StageCfg<N, PDE, MIXED, EB, LM>;
+template <int N, int PDE, bool MIXED, int EB, int LM, bool MD = false>
+__global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_stage(StageArgs A) {
+    using Cfg = StageCfg<N, PDE, MIXED, EB, LM, MD>;
     using T = typename Cfg::T;
     using P = Pde<PDE>;
     constexpr int F = Cfg::F, NP = Cfg::NP, NFP = Cfg::NFP, NT = Cfg::NT;
     __shared__ T s1[Cfg::R1];
     __shared__ T s2[Cfg::R2];
+    __shared__ T s3[MD ? Cfg::R3 : 1];
     __shared__ uint4 tab16[Cfg::TAB16];
     const uint16_t *tab = reinterpret_cast<const uint16_t *>(tab16);
     __shared__ double lw[N + 1];  // l_0 = 1, l_1 .. l_N (src/constants.jl:10-12)
@@ -219,11 +239,28 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
         if (t < ne * 4 * NFP) {
             const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, m = t % NFP;
             cnv[r] = A.conn[(e0 + e2) * 4 + phi];
-            if (((cnv[r].y >> 5) & 7) == 0) {
+            if (((cnv[r].y >> 5) & 7) == 0 && (!MD || ((cnv[r].y >> 8) & 15) == N)) {
                 const int nvi = nvid[((cnv[r].y & 3) * 6 + ((cnv[r].y >> 2) & 7)) * NFP + m];
-                const double *qn = A.q_in + (int64_t)cnv[r].x * F * NP + nvi;
+                const double *qn = MD ? A.q_nb + A.nboff[(e0 + e2) * 4 + phi] + nvi
+                                      : A.q_in + (int64_t)cnv[r].x * F * NP + nvi;
 #pragma unroll
                 for (int f = 0; f < F; ++f) nb[r][f] = T(qn[f * NP]);
+            }
+        }
+    }
+    if constexpr (MD) {
+        // whole face traces of neighbours of another degree -> s3[(element, face, field)][node]
+        for (int t = tid; t < ne * 4 * kTraceMax; t += NT) {
+            const int e2 = t / (4 * kTraceMax), phi = (t / kTraceMax) & 3, jn = t % kTraceMax;
+            const int2 cn = A.conn[(e0 + e2) * 4 + phi];
+            const int Nn = (cn.y >> 8) & 15;
+            if (((cn.y >> 5) & 7) == 0 && Nn != N && jn < nfp2(Nn)) {
+                int a, b;
+                decode2(Nn, jn, a, b);
+                const int npn = np3(Nn);
+                const double *qn = A.q_nb + A.nboff[(e0 + e2) * 4 + phi] + face_vidx(Nn, cn.y & 3, a, b);
+#pragma unroll
+                for (int f = 0; f < F; ++f) s3[((e2 * 4 + phi) * F + f) * kTraceMax + jn] = T(qn[f * npn]);
             }
         }
     }
@@ -264,7 +301,27 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
             T um[F], d[F], o[F];
 #pragma unroll
             for (int f = 0; f < F; ++f) um[f] = s1[(e2 * F + f) * NP + vi];
-            if (((cnv[r].y >> 5) & 7) == 0) {
+            if (MD && ((cnv[r].y >> 5) & 7) == 0 && ((cnv[r].y >> 8) & 15) != N) {
+                // the neighbour's trace of degree Nn brought to degree N (in the neighbour's
+                // face orientation), read at this node through the face permutation
+                const int Nn = (cnv[r].y >> 8) & 15, p = (cnv[r].y >> 2) & 7, nfn = nfp2(Nn);
+                const int ab = dec[np3(N - 1) + m];
+                const int a = ab & 255, b = ab >> 8, c = N - a - b;
+                const int q0 = perm_entry(p, 0), q1 = perm_entry(p, 1);
+                const int a2 = q0 == 0 ? a : (q0 == 1 ? b : c), b2 = q1 == 0 ? a : (q1 == 1 ? b : c);
+                const double *Tr = A.tr + A.troff[11 * N + Nn] + (int64_t)lin2(N, a2, b2) * nfn;
+                const T *tn = s3 + (e2 * 4 + phi) * F * kTraceMax;
+                T up[F];
+#pragma unroll
+                for (int f = 0; f < F; ++f) up[f] = T(0);
+                for (int jn = 0; jn < nfn; ++jn) {
+                    const T w = T(Tr[jn]);
+#pragma unroll
+                    for (int f = 0; f < F; ++f) up[f] += w * tn[f * kTraceMax + jn];
+                }
+#pragma unroll
+                for (int f = 0; f < F; ++f) d[f] = up[f] - um[f];
+            } else if (((cnv[r].y >> 5) & 7) == 0) {
 #pragma unroll
                 for (int f = 0; f < F; ++f) d[f] = nb[r][f] - um[f];
             } else {
@@ -436,7 +493,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM>::NT)) void k_stage
             if (A.rhs_only) {
                 res[g] = TR(rhs);
             } else {
-                const double r = A.first ? A.dt * rhs : A.ca * (double)rv[f] + A.dt * rhs;
+                const double dte = MD ? A.dt * A.dte[eg] : A.dt;
+                const double r = A.first ? dte * rhs : A.ca * (double)rv[f] + dte * rhs;
                 const TR rs = TR(r);
                 res[g] = rs;
                 A.q_out[g] = qv[f] + A.cb * (double)rs;
@@ -452,6 +510,9 @@ int stage_launch_wave_f64(int N, int tiling, const StageArgs &a, hipStream_t st,
 int stage_launch_wave_mixed(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
 int stage_launch_maxwell_f64(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
 int stage_launch_maxwell_mixed(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
+// elements of mixed degree (wave system)
+int stage_launch_wave_md_f64(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
+int stage_launch_wave_md_mixed(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe);
 
 constexpr int kStageMaxN = 10;
 // candidate tilings (elements per block, lift organisation), in autotuning order
@@ -490,6 +551,40 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             one(std::integral_constant<int, 1>{}, M2{});                                               \
             one(std::integral_constant<int, 2>{}, M2{});                                               \
             one(std::integral_constant<int, 4>{}, M2{});                                               \
+            one(std::integral_constant<int, 1>{}, G0{});                                               \
+            one(std::integral_constant<int, 2>{}, G0{});                                               \
+            one(std::integral_constant<int, 4>{}, G0{});                                               \
+            one(std::integral_constant<int, 1>{}, P1{});                                               \
+            one(std::integral_constant<int, 2>{}, P1{});                                               \
+            one(std::integral_constant<int, 4>{}, P1{});                                               \
+            return rc;                                                                                 \
+        });                                                                                            \
+    }                                                                                                  \
+    }
+
+#define BB_STAGE_DEFINE_MD(NAME, PDE, MIXED)                                                           \
+    namespace bb {                                                                                     \
+    int NAME(int N, int tiling, const StageArgs &a, hipStream_t st, bool probe) {                     \
+        const int64_t K = a.K;                                                                         \
+        return dispatch_n<1, kStageMaxN>(N, [&](auto nc) -> int {                                      \
+            constexpr int NN = decltype(nc)::value;                                                    \
+            int rc = BB_ERR_UNSUPPORTED;                                                               \
+            auto one = [&](auto ebc, auto lmc) {                                                       \
+                constexpr int E = decltype(ebc)::value, LMV = decltype(lmc)::value;                    \
+                using Cfg = StageCfg<NN, PDE, MIXED, E, LMV, true>;                                    \
+                if constexpr (Cfg::valid) {                                                            \
+                    if (tiling == tiling_code(E, LMV)) {                                               \
+                        if (probe) {                                                                   \
+                            rc = BB_OK;                                                                \
+                        } else {                                                                       \
+                            k_stage<NN, PDE, MIXED, E, LMV, true><<<grid1d(K, E), Cfg::NT, 0, st>>>(a); \
+                            rc = hipGetLastError() == hipSuccess ? BB_OK : BB_ERR_HIP;                 \
+                        }                                                                              \
+                    }                                                                                  \
+                }                                                                                      \
+            };                                                                                         \
+            using G0 = std::integral_constant<int, LIFT_GATHER>;                                       \
+            using P1 = std::integral_constant<int, LIFT_PINGPONG>;                                     \
             one(std::integral_constant<int, 1>{}, G0{});                                               \
             one(std::integral_constant<int, 2>{}, G0{});                                               \
             one(std::integral_constant<int, 4>{}, G0{});                                               \

@@ -63,6 +63,7 @@ typedef struct bb_layout {
 
 typedef struct bb_ctx bb_ctx;
 typedef struct bb_solver bb_solver;
+typedef struct bb_batch bb_batch;
 
 /* ------------------------------------------------------------------ context */
 int bb_ctx_create(int device, bb_ctx **out);
@@ -195,6 +196,35 @@ int bb_solver_set_tuning(bb_solver *s, int prec, int tiling);
 /* HBM bytes one fused stage moves by the algorithmic count (state in/out, residual
  * in/out, neighbour traces, geometry, connectivity) and the DOF count K*Np*F */
 int bb_solver_stage_bytes(bb_solver *s, int prec, double *bytes, double *dofs);
+
+/* ---- batches of elements of mixed degree ------------------------------------------
+ * No single counterpart in the reference, whose examples run one mesh at one degree
+ * (examples/3d_wave.jl:36-134); this is that RHS + LSRK45 for many independent meshes
+ * at once, each element with its own degree 1 <= N <= 10 (wave system, F = 4).
+ * Elements are numbered grouped by degree (deg non-decreasing); the state is the
+ * concatenation over degree groups of [K_N][F][Np(N)] FP64.  nbr holds neighbours in
+ * that numbering; nbr_face, perm, bc, nJ, geo, J as for bb_solver_create (int32 arrays);
+ * dt [K] is each element's step (one per mesh), scaled by the dt_scale of a step call.
+ * A neighbour trace of degree Nn != N is brought to degree N by the nfp(N) x nfp(Nn)
+ * row-major matrix at transfer + transfer_off[11 N + Nn] (degree elevation for Nn < N,
+ * the L2 projection for Nn > N), then the upwind flux is that of bb_solver_create.
+ * Each stage launches the fused kernel once per degree present. */
+int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_t *deg_host, const double *geo_host,
+                    const double *J_host, const double *nJ_host, const int32_t *nbr_host,
+                    const int32_t *nbr_face_host, const int32_t *perm_host, const int32_t *bc_host,
+                    const double *bc_sign_host, const double *dt_host, const double *transfer_host,
+                    const int64_t *transfer_off_host /* [11][11] */, int64_t transfer_len, bb_batch **out);
+int bb_batch_destroy(bb_batch *b);
+int bb_batch_state_words(bb_batch *b, int64_t *words);
+int bb_batch_set_state(bb_batch *b, const double *q_host);
+int bb_batch_get_state(bb_batch *b, double *q_host);
+int bb_batch_rhs(bb_batch *b, int prec, double *du_host);
+int bb_batch_step(bb_batch *b, int prec, double dt_scale, int nsteps);
+int bb_batch_graph_capture(bb_batch *b, int prec, double dt_scale, int nsteps);
+int bb_batch_graph_launch(bb_batch *b);
+/* tilings[N] (11 entries) receives the chosen tiling code of each degree present */
+int bb_batch_autotune(bb_batch *b, int prec, int *tilings);
+int bb_batch_set_tuning(bb_batch *b, int prec, int N, int tiling);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,10 @@ def lib():
         L.orc_lsrk_coeffs.argtypes = [dp, dp]
         L.orc_face_lift_ld_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
         L.orc_flux.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, i32p, i8p, dp, dp, dp, dp, dp]
+        L.orc_rhs_md.argtypes = [C.c_int, C.c_int, C.c_int64, i32p, i64p, dp, i32p, i32p, i32p, i32p, dp, dp, dp, dp,
+                                 dp, i64p, dp, C.c_int]
+        L.orc_lsrk_step_md.argtypes = [C.c_int, C.c_int, C.c_int64, i32p, i64p, dp, dp, dp, i32p, i32p, i32p, i32p,
+                                       dp, dp, dp, dp, dp, i64p, C.c_int]
         _lib = L
     return _lib
 
@@ -236,3 +240,45 @@ def lsrk_step(pde, N, q_aos, dt, ma: MeshArrays, nthreads=1, flux_kind=1):
                         ma.bc.reshape(-1), ma.bc_sign, ma.nx.reshape(-1), ma.ny.reshape(-1), ma.nz.reshape(-1),
                         ma.geo.reshape(-1), ma.J, nthreads)
     return q_aos
+
+
+class BatchArrays:
+    """Oracle input for elements of mixed degree: a bbdg.batch.Batch (host set-up data,
+    device element order) plus the degree-transfer tables, offsets in a 21 x 21 table."""
+
+    def __init__(self, batch, bc_sign):
+        from bbdg import transfer as TR  # host set-up tables (exact rationals, rounded once)
+
+        self.b = batch
+        self.K = batch.K
+        self.deg = np.ascontiguousarray(batch.deg.astype(np.int32))
+        self.qoff = np.ascontiguousarray(batch.qoff.astype(np.int64))
+        self.nbr = np.ascontiguousarray(batch.nbr.astype(np.int32).reshape(-1))
+        self.nbf = np.ascontiguousarray(batch.nbr_face.astype(np.int32).reshape(-1))
+        self.perm = np.ascontiguousarray(batch.perm.astype(np.int32).reshape(-1))
+        self.bc = np.ascontiguousarray(batch.bc.astype(np.int32).reshape(-1))
+        self.nJ = np.ascontiguousarray(batch.nJ.reshape(-1))
+        self.geo = np.ascontiguousarray(batch.geo.reshape(-1))
+        self.J = np.ascontiguousarray(batch.J)
+        self.dte = np.ascontiguousarray(batch.dte)
+        T, off = TR.pack(21)
+        self.T = np.ascontiguousarray(T)
+        self.toff = np.ascontiguousarray(off.reshape(-1))
+        self.bc_sign = np.ascontiguousarray(np.asarray(bc_sign, dtype=np.float64))
+
+
+def rhs_md(pde, q_node_major, ba: BatchArrays, nthreads=1, flux_kind=1):
+    """mixed-degree RHS on a per-element [Np][F] state (Batch.to_node_major layout)"""
+    q = np.ascontiguousarray(q_node_major, dtype=np.float64)
+    du = np.zeros_like(q)
+    lib().orc_rhs_md(pde, flux_kind, ba.K, ba.deg, ba.qoff, q, ba.nbr, ba.nbf, ba.perm, ba.bc, ba.bc_sign, ba.nJ,
+                     ba.geo, ba.J, ba.T, ba.toff, du, nthreads)
+    return du
+
+
+def lsrk_step_md(pde, q_node_major, ba: BatchArrays, dt_scale=1.0, nthreads=1, flux_kind=1):
+    """one LSRK45 step in place, element e with step dt_scale * dt_e"""
+    res = np.zeros_like(q_node_major)
+    dt = np.ascontiguousarray(ba.dte * dt_scale)
+    lib().orc_lsrk_step_md(pde, flux_kind, ba.K, ba.deg, ba.qoff, q_node_major, res, dt, ba.nbr, ba.nbf, ba.perm,
+                           ba.bc, ba.bc_sign, ba.nJ, ba.geo, ba.J, ba.T, ba.toff, nthreads)

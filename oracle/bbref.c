@@ -500,19 +500,19 @@ void orc_interface_flux(const orc_mesh_t *M, const double *uM, double *flux) {
 }
 
 /* per-element part of examples/3d_wave.jl:58-80 (wave) / 3d_maxwell.jl:211-233 (maxwell),
- * then du ./= J[e] (:83 / :236).  scratch: 12*Np*F + lift scratch doubles. */
-static void rhs_element(const orc_mesh_t *M, int64_t e, const double *q, const double *flux,
-                        double *du, double *scratch) {
-    int N = M->N, F = M->F, np = orc_np(N), nfp = orc_nfp(N);
+ * then du ./= J[e] (:83 / :236), on one element of degree N: qe, due [Np][F], fe [4Nfp][F],
+ * g the element's 9 geometric factors.  scratch: 12*Np*F + lift scratch doubles. */
+static void rhs_element_core(int pde, int N, int F, const double *qe, const double *fe, double *due,
+                             const double *g, double Je, double *scratch) {
+    int np = orc_np(N), nfp = orc_nfp(N);
     double *fxu = scratch, *fyu = fxu + np * F, *fzu = fyu + np * F;
     double *d[9];
     for (int m = 0; m < 9; ++m) d[m] = fzu + np * F + m * np * F;
     double *ls = d[8] + np * F;
     double *xcol = ls + orc_lift_scratch_len(N);      /* 4Nfp */
     double *ocol = xcol + 4 * nfp;                     /* Np */
-    const double *qe = q + e * np * F;
     for (int n = 0; n < np; ++n) {
-        if (M->pde == 0) {
+        if (pde == 0) {
             wave_fx(qe + n * F, fxu + n * F);
             wave_fy(qe + n * F, fyu + n * F);
             wave_fz(qe + n * F, fzu + n * F);
@@ -528,14 +528,11 @@ static void rhs_element(const orc_mesh_t *M, int64_t e, const double *q, const d
         for (int ax = 0; ax < 3; ++ax)
             for (int f = 0; f < F; ++f) orc_deriv_mul_strided(ax, N, fu[c] + f, d[3 * c + ax] + f, F);
     /* mul!(view(du,:,e), LIFT, view(interface_flux,:,e)) on SVector{F}: per field */
-    double *due = du + e * np * F;
-    const double *fe = flux + e * 4 * nfp * F;
     for (int f = 0; f < F; ++f) {
         for (int m = 0; m < 4 * nfp; ++m) xcol[m] = fe[m * F + f];
         orc_lift(N, xcol, ocol, ls);
         for (int n = 0; n < np; ++n) due[n * F + f] = ocol[n];
     }
-    const double *g = M->geo + e * 9;
     double rxJ = g[0], sxJ = g[1], txJ = g[2], ryJ = g[3], syJ = g[4], tyJ = g[5], rzJ = g[6],
            szJ = g[7], tzJ = g[8];
     for (int n = 0; n < np; ++n)
@@ -543,11 +540,17 @@ static void rhs_element(const orc_mesh_t *M, int64_t e, const double *q, const d
             int o = n * F + f;
             double s = rxJ * d[0][o] + sxJ * d[1][o] + txJ * d[2][o] + ryJ * d[3][o] + syJ * d[4][o] +
                        tyJ * d[5][o] + rzJ * d[6][o] + szJ * d[7][o] + tzJ * d[8][o];
-            if (M->pde == 0) due[o] -= s;
+            if (pde == 0) due[o] -= s;
             else due[o] += s;
         }
-    double Je = M->J[e];
     for (int o = 0; o < np * F; ++o) due[o] = due[o] / Je;
+}
+
+static void rhs_element(const orc_mesh_t *M, int64_t e, const double *q, const double *flux,
+                        double *du, double *scratch) {
+    int np = orc_np(M->N), nfp = orc_nfp(M->N);
+    rhs_element_core(M->pde, M->N, M->F, q + e * np * M->F, flux + e * 4 * nfp * M->F, du + e * np * M->F,
+                     M->geo + e * 9, M->J[e], scratch);
 }
 
 static int rhs_scratch_len(int N, int F) {
@@ -587,6 +590,103 @@ void orc_rhs(int pde, int flux_kind, int N, int64_t K, const double *q, const in
     }
     free(uM);
     free(flux);
+}
+
+/* ------------------------------------------------ elements of mixed degree */
+
+/* the 6 permutations of {0,1,2} in lexicographic order */
+static const int PERM3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+/* Elements that carry their own degree (many meshes batched, each element of degree
+ * deg[e] <= 20).  No counterpart in the reference, whose RHS has one degree; built from its
+ * pieces: the face trace of the neighbour, of degree Nn, is brought to the element's
+ * degree N by the matrix T_{N,Nn} (degree elevation if Nn < N, the L2 projection onto
+ * degree N if Nn > N; T + toff[N * 21 + Nn], row-major nfp(N) x nfp(Nn); toff < 0 or
+ * Nn == N: identity), in the neighbour's own face orientation, and then read through the
+ * face permutation exactly as mapP does for equal degrees.  The interface flux, volume
+ * term and lift are those of orc_rhs at degree N.
+ *   q, du : element e's [Np(N_e)][F] block at qoff[e] (doubles)
+ *   nbr, nbf, perm, bc : [K][4] neighbour, its face, permutation code, 0 interior / 1 wall
+ *   nJ : [K][4][3], geo : [K][9], J : [K]  */
+void orc_rhs_md(int pde, int flux_kind, int64_t K, const int32_t *deg, const int64_t *qoff, const double *q,
+                const int32_t *nbr, const int32_t *nbf, const int32_t *perm, const int32_t *bc,
+                const double *bc_sign, const double *nJ, const double *geo, const double *J, const double *T,
+                const int64_t *toff, double *du, int nthreads) {
+    int F = pde == 0 ? 4 : 6;
+    int nmax = 1;
+    for (int64_t e = 0; e < K; ++e)
+        if (deg[e] > nmax) nmax = deg[e];
+    int nf_max = orc_nfp(nmax);
+    /* per thread: rhs scratch, element flux [4Nfp][F], neighbour trace, transferred trace,
+     * two fmask tables */
+    int64_t sl = rhs_scratch_len(nmax, F) + 4 * nf_max * F + 2 * nf_max * F;
+    int64_t il = 8 * nf_max;
+#ifdef _OPENMP
+    int nt = nthreads > 1 ? nthreads : 1;
+#else
+    int nt = 1;
+#endif
+    double *scr = (double *)malloc(sizeof(double) * sl * nt);
+    int32_t *iscr = (int32_t *)malloc(sizeof(int32_t) * il * nt);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (int64_t e = 0; e < K; ++e) {
+#ifdef _OPENMP
+        int th = omp_get_thread_num();
+#else
+        int th = 0;
+#endif
+        double *s = scr + sl * th, *fe = s + rhs_scratch_len(nmax, F), *tn = fe + 4 * nf_max * F,
+               *tt = tn + nf_max * F;
+        int32_t *fm = iscr + il * th, *fmn = fm + 4 * nf_max;
+        int N = deg[e], nfp = orc_nfp(N);
+        const double *qe = q + qoff[e];
+        orc_fmask(N, fm);
+        for (int f = 0; f < 4; ++f) {
+            int64_t t = e * 4 + f;
+            const double *nj = nJ + t * 3;
+            const double *up = NULL;
+            if (!bc[t]) {
+                int nb = nbr[t], Nn = deg[nb], nfpn = orc_nfp(Nn);
+                orc_fmask(Nn, fmn);
+                const double *qn = q + qoff[nb];
+                for (int j = 0; j < nfpn; ++j)
+                    for (int c = 0; c < F; ++c) tn[j * F + c] = qn[(int64_t)fmn[nbf[t] * nfpn + j] * F + c];
+                if (Nn == N || toff[N * 21 + Nn] < 0) {
+                    up = tn;
+                } else {
+                    const double *Tm = T + toff[N * 21 + Nn];
+                    for (int r = 0; r < nfp; ++r)
+                        for (int c = 0; c < F; ++c) {
+                            double v = 0.0;
+                            for (int j = 0; j < nfpn; ++j) v += Tm[(int64_t)r * nfpn + j] * tn[j * F + c];
+                            tt[r * F + c] = v;
+                        }
+                    up = tt;
+                }
+            }
+            int m = 0;
+            for (int b = 0; b <= N; ++b)
+                for (int a = 0; a <= N - b; ++a, ++m) {
+                    const double *um = qe + (int64_t)fm[f * nfp + m] * F;
+                    double d[6], o[6];
+                    if (bc[t]) {
+                        for (int c = 0; c < F; ++c) d[c] = bc_sign[c] * um[c] - um[c];
+                    } else {
+                        int abc[3] = {a, b, N - a - b}, p = perm[t];
+                        int a2 = abc[PERM3[p][0]], b2 = abc[PERM3[p][1]];
+                        int mm = a2 + b2 * (2 * N + 3 - b2) / 2;
+                        for (int c = 0; c < F; ++c) d[c] = up[mm * F + c] - um[c];
+                    }
+                    node_flux(pde, flux_kind, d, nj[0], nj[1], nj[2], o);
+                    for (int c = 0; c < F; ++c) fe[(f * nfp + m) * F + c] = o[c];
+                }
+        }
+        rhs_element_core(pde, N, F, qe, fe, du + qoff[e], geo + e * 9, J[e], s);
+    }
+    free(scr);
+    free(iscr);
 }
 
 /* ----------------------------------------------------------- time stepping */
@@ -686,4 +786,27 @@ void orc_face_lift_ld_cols(int N, int64_t ncols, const double *x, double *out) {
         #undef RED_LD
     }
     free(E);
+}
+
+/* one LSRK45 step of the mixed-degree system; element e advances with its own step dt[e]
+ * (many independent meshes, each with its own step).  res zeroed on entry. */
+void orc_lsrk_step_md(int pde, int flux_kind, int64_t K, const int32_t *deg, const int64_t *qoff, double *q,
+                      double *res, const double *dt, const int32_t *nbr, const int32_t *nbf, const int32_t *perm,
+                      const int32_t *bc, const double *bc_sign, const double *nJ, const double *geo,
+                      const double *J, const double *T, const int64_t *toff, int nthreads) {
+    int F = pde == 0 ? 4 : 6;
+    int64_t n = qoff[K - 1] + (int64_t)orc_np(deg[K - 1]) * F;
+    double *du = (double *)malloc(sizeof(double) * n);
+    memset(res, 0, sizeof(double) * n);
+    for (int s = 0; s < 5; ++s) {
+        orc_rhs_md(pde, flux_kind, K, deg, qoff, q, nbr, nbf, perm, bc, bc_sign, nJ, geo, J, T, toff, du, nthreads);
+        for (int64_t e = 0; e < K; ++e) {
+            int64_t w = (int64_t)orc_np(deg[e]) * F;
+            for (int64_t m = qoff[e]; m < qoff[e] + w; ++m) {
+                res[m] = LSRK_A[s] * res[m] + dt[e] * du[m];
+                q[m] = q[m] + LSRK_B[s] * res[m];
+            }
+        }
+    }
+    free(du);
 }
