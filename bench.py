@@ -238,6 +238,68 @@ def lift_study(ctx, K=65536):
     return rows
 
 
+def batch_bench(ctx, steps=100, n_inst=64, seed=5):
+    """Batching workload: n_inst independent acoustic-wave instances, each a jittered
+    periodic Kuhn mesh of n^3 x 6 tets with n ~ U{4..16}; every element has its own
+    degree N ~ U{3..10}; Gaussian pulses p = exp(-|x - c|^2 / w^2) with c ~ U[0.3,0.7]^3,
+    w ~ U[0.05,0.2], u = v = w = 0 (all from seed 5); per-instance step 0.5 h / 10^2.
+    Elements are grouped by degree; every stage launches the fused kernel once per degree
+    group, a step is one hipGraph replay, tilings are autotuned per group.  Timed: steps
+    LSRK45 steps in mixed precision and in FP64; errors of mixed against FP64 after them."""
+    from bbdg import mesh
+    from bbdg.batch import Batch, BatchSolver
+
+    t0 = time.time()
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(4, 17, n_inst)
+    meshes = [mesh.kuhn_mesh(int(n), 0.1, seed * 1000 + i, periodic=True) for i, n in enumerate(ns)]
+    degrees = [rng.integers(3, 11, m.K) for m in meshes]
+    fns = []
+    for i in range(n_inst):
+        c, w = rng.uniform(0.3, 0.7, 3), rng.uniform(0.05, 0.2)
+        fns.append(lambda x, y, z, c=c, w=w: np.stack(
+            [np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / w ** 2), 0 * x, 0 * x, 0 * x]))
+    b = Batch(meshes, degrees, [0.5 * m.h / 10 ** 2 for m in meshes])
+    q0 = b.interpolate(fns)
+    log(f"batch: {n_inst} instances, K={b.K}, DOF={b.dofs}, groups {[(N, c) for N, _, c in b.groups()]}, "
+        f"set-up {time.time() - t0:.1f}s")
+    s = BatchSolver(b, ctx=ctx)
+    out = {"instances": n_inst, "K": b.K, "dofs": b.dofs, "steps": steps,
+           "elements_per_degree": {int(N): int(c) for N, _, c in b.groups()}}
+    states = {}
+    for prec in ("mixed", "f64"):
+        s.set_state(q0)
+        tun = s.autotune(prec)
+        s.set_state(q0)
+        s.capture(1, prec)
+        s.launch()
+        ctx.sync()
+        s.set_state(q0)
+        ctx.timer_start()
+        for _ in range(steps):
+            s.launch()
+        ms = ctx.timer_stop() / steps
+        states[prec] = s.get_state()
+        out[prec] = {"ms_per_step": ms, "dof_updates_per_s": b.dofs / (ms * 1e-3), "tilings": tun}
+        log(f"batch {prec}: {ms:.3f} ms/step, {b.dofs / ms * 1e3:.3e} DOF/s, tilings {tun}")
+    d = states["mixed"] - states["f64"]
+    ref = states["f64"]
+    per = {}
+    for f, nm in enumerate(["p", "u", "v", "w"]):
+        num = den = mx = 0.0
+        for N, st, c in b.groups():
+            dv, rv = b.field_view(d, N)[:, f], b.field_view(ref, N)[:, f]
+            num += float((dv ** 2).sum())
+            den += float((rv ** 2).sum())
+            mx = max(mx, float(np.abs(dv).max()))
+        per[nm] = {"l2_rel_to_state": num ** 0.5 / float(np.linalg.norm(ref)), "max_abs": mx}
+    out["errors_mixed_vs_f64"] = {"rel_l2_all": float(np.linalg.norm(d) / np.linalg.norm(ref)), "fields": per}
+    out["speedup_mixed_vs_f64"] = out["f64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    log(f"batch mixed vs f64 after {steps} steps: rel L2 {out['errors_mixed_vs_f64']['rel_l2_all']:.3e}")
+    s.close()
+    return out
+
+
 def cpu_baseline(n_sample, steps, threads):
     import oracle as O
     from bbdg import basis, mesh
@@ -273,6 +335,9 @@ def main():
     ap.add_argument("--lift-study", action="store_true",
                     help="run only the lift conditioning study (single-face lift precision variants)")
     ap.add_argument("--ops-only", action="store_true", help="run only the operator microbenchmark")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-degree workload")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched mixed-degree workload")
+    ap.add_argument("--batch-steps", type=int, default=100, help="timed steps of the batched workload")
     ap.add_argument("--stage-only", action="store_true",
                     help="run only a few fused mixed stages (for counter collection)")
     args = ap.parse_args()
@@ -287,11 +352,18 @@ def main():
     if args.lift_study:
         print(json.dumps({"lift_conditioning": lift_study(bbdg.default_context())}))
         return
+    if args.batch_only:
+        ctx = bbdg.default_context()
+        print(json.dumps({"batch_mixed_degree": batch_bench(ctx, steps=args.batch_steps)}))
+        stream_bandwidth(ctx, gib=1.0, reps=1)  # the calibration dispatch for HBM byte counters
+        return
     if args.ops_only:
-        ops = operator_bench(bbdg.default_context())
+        ctx = bbdg.default_context()
+        ops = operator_bench(ctx)
         for k, v in ops.items():
             log(f"operator {k}: {v['us']:.1f} us, {v['GBs']:.0f} GB/s ({v['frac_of_8TBs']:.2f} of 8 TB/s)")
         print(json.dumps({"operators_N7_K262144_F4": ops}))
+        stream_bandwidth(ctx, gib=1.0, reps=1)  # the calibration dispatch for HBM byte counters
         return
     N, n = 9, args.n
     t0 = time.time()
@@ -361,6 +433,9 @@ def main():
         ops = operator_bench(bbdg.default_context())
         for k, v in ops.items():
             log(f"operator {k}: {v['us']:.1f} us, {v['GBs']:.0f} GB/s ({v['frac_of_8TBs']:.2f} of 8 TB/s)")
+    batch = None
+    if not args.no_batch:
+        batch = batch_bench(bbdg.default_context(), steps=args.batch_steps)
     cpu = None
     if not args.no_cpu:
         thr = min(16, os.cpu_count() or 1)
@@ -402,6 +477,7 @@ def main():
                      "stream_copy_GBs": copy_gbs, "frac_of_stream_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
         "operators_N7_K262144_F4": ops,
+        "batch_mixed_degree": batch,
     }
     print(json.dumps(out))
 
