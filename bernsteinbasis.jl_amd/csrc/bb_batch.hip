@@ -84,7 +84,7 @@ static int batch_launch(bb_batch *b, const BatchGroup &g, int prec, int tiling, 
 
 static int batch_default_tiling(int prec, int N) {
     const int NP = np3(N);
-    for (int lm : {LIFT_GATHER, LIFT_PINGPONG}) {
+    for (int lm : {LIFT_GATHER, LIFT_PINGPONG, LIFT_WAVE}) {
         int best = 0;
         for (int e : kStageEB) {
             if (e * NP > 1024) break;
@@ -390,7 +390,7 @@ int bb_batch_autotune(bb_batch *b, int prec, int *tilings) {
         a.first = 0;
         float best = 1e30f;
         int best_t = 0;
-        for (int lm : {LIFT_GATHER, LIFT_PINGPONG})
+        for (int lm : {LIFT_GATHER, LIFT_PINGPONG, LIFT_WAVE})
             for (int e : kStageEB) {
                 const int t = tiling_code(e, lm);
                 if (!batch_valid(prec, g.N, t)) continue;

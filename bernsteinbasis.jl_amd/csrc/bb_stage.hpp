@@ -74,7 +74,20 @@ constexpr int kTraceMax = 66;  // face nodes of the highest degree of a fused ke
 //             as one product on the matrix cores: L0 (Nfp x Nfp, zero-padded to 16-row
 //             tiles and 4-deep k steps) times the chain columns (element, field, face) in
 //             v_mfma_f32_16x16x4f32 tiles with FP32 accumulation.
-enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1, LIFT_GATHER_MFMA = 2 };
+//  WAVE     : GATHER, with each (element, face) chain owned by one wave from the flux to
+//             its last reduction step, so that these phases need only wave-level ordering
+//             (no workgroup barrier between them); the element state gets its own LDS
+//             buffer, since fast waves overwrite chain slabs while slow ones still read it.
+enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1, LIFT_GATHER_MFMA = 2, LIFT_WAVE = 3 };
+
+// LDS written by one lane and read by another lane of the same wave: a wave executes its
+// LDS instructions in order; the fences keep the compiler from moving accesses across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = false> struct StageCfg {
     using T = std::conditional_t<MIXED, float, double>;
@@ -83,14 +96,16 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int NT = ((EB * NP + 63) / 64) * 64;
     // PINGPONG: s1 = state, then chain A; s2 = flux, then chain B
     // GATHER:   s1 = state, then all chain vectors (EB F 4 Np); s2 = flux
-    static constexpr bool GATHER = LM == LIFT_GATHER || LM == LIFT_GATHER_MFMA;
+    static constexpr bool GATHER = LM == LIFT_GATHER || LM == LIFT_GATHER_MFMA || LM == LIFT_WAVE;
+    static constexpr bool WAVEF = LM == LIFT_WAVE;
+    static constexpr int R0 = WAVEF ? EB * F * NP : 0;  // separate state buffer (WAVE)
     static constexpr int R1 = GATHER ? EB * F * 4 * NP
                               : ((EB * F * NP > EB * F * 4 * NFP) ? EB * F * NP : EB * F * 4 * NFP);
     static constexpr int R2 = EB * F * 4 * NFP;
     static constexpr int TAB = 2 * NP + 28 * NFP;  // ijk, dec, own face nodes, neighbour face nodes
     static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
     static constexpr int R3 = MD ? EB * 4 * F * kTraceMax : 0;  // neighbour traces of other degrees
-    static constexpr size_t lds = (size_t)(R1 + R2 + R3) * sizeof(T) + (size_t)TAB16 * 16 + (N + 1) * 8;
+    static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (size_t)TAB16 * 16 + (N + 1) * 8;
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
     static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024 && (LM != LIFT_GATHER_MFMA || MIXED);
@@ -177,6 +192,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     constexpr int F = Cfg::F, NP = Cfg::NP, NFP = Cfg::NFP, NT = Cfg::NT;
     __shared__ T s1[Cfg::R1];
     __shared__ T s2[Cfg::R2];
+    __shared__ T s0[Cfg::WAVEF ? Cfg::R0 : 1];
+    T *sst = Cfg::WAVEF ? s0 : s1;  // the element state in LDS
     __shared__ T s3[MD ? Cfg::R3 : 1];
     __shared__ uint4 tab16[Cfg::TAB16];
     const uint16_t *tab = reinterpret_cast<const uint16_t *>(tab16);
@@ -229,15 +246,32 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         }
     }
     __syncthreads();  // tables visible
-    // face-node work items of this thread (at most RF of them)
-    constexpr int RF = (EB * 4 * NFP + NT - 1) / NT;
+    // face-node work items of this thread (at most RF of them).  WAVE: chain c = (element,
+    // face) belongs to wave c mod NW, its nodes to the lanes; otherwise items are dealt to
+    // the threads in order.
+    constexpr int MI = (NFP + 63) / 64, CW = (EB * 4 + NW - 1) / NW;
+    constexpr int RF = Cfg::WAVEF ? CW * MI : (EB * 4 * NFP + NT - 1) / NT;
+    auto fitem = [&](int r, int &e2, int &phi, int &m) -> bool {
+        if constexpr (Cfg::WAVEF) {
+            const int c = wave_u + NW * (r / MI);
+            m = (r % MI) * 64 + (tid & 63);
+            e2 = c >> 2;
+            phi = c & 3;
+            return c < ne * 4 && m < NFP;
+        } else {
+            const int t = tid + r * NT;
+            e2 = t / (4 * NFP);
+            phi = (t / NFP) & 3;
+            m = t % NFP;
+            return t < ne * 4 * NFP;
+        }
+    };
     T nb[RF][F];
     int2 cnv[RF];
 #pragma unroll
     for (int r = 0; r < RF; ++r) {
-        const int t = tid + r * NT;
-        if (t < ne * 4 * NFP) {
-            const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, m = t % NFP;
+        int e2, phi, m;
+        if (fitem(r, e2, phi, m)) {
             cnv[r] = A.conn[(e0 + e2) * 4 + phi];
             if (((cnv[r].y >> 5) & 7) == 0 && (!MD || ((cnv[r].y >> 8) & 15) == N)) {
                 const int nvi = nvid[((cnv[r].y & 3) * 6 + ((cnv[r].y >> 2) & 7)) * NFP + m];
@@ -266,7 +300,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     }
     if (own) {
 #pragma unroll
-        for (int f = 0; f < F; ++f) s1[(el * F + f) * NP + n] = T(qv[f]);
+        for (int f = 0; f < F; ++f) sst[(el * F + f) * NP + n] = T(qv[f]);
     }
     __syncthreads();  // state visible
 
@@ -283,7 +317,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             T dr, ds, dt;
-            grad_ref<N, T>(s1 + (el * F + f) * NP, n, i, j, k, dr, ds, dt);
+            grad_ref<N, T>(sst + (el * F + f) * NP, n, i, j, k, dr, ds, dt);
             const T gx = g0 * dr + g1 * ds + g2 * dt;
             const T gy = g3 * dr + g4 * ds + g5 * dt;
             const T gz = g6 * dr + g7 * ds + g8 * dt;
@@ -294,13 +328,12 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     // ---- interface flux at every face node of the block's elements -> s2
 #pragma unroll
     for (int r = 0; r < RF; ++r) {
-        const int t = tid + r * NT;
-        if (t < ne * 4 * NFP) {
-            const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, m = t % NFP;
+        int e2, phi, m;
+        if (fitem(r, e2, phi, m)) {
             const int vi = fvid[phi * NFP + m];
             T um[F], d[F], o[F];
 #pragma unroll
-            for (int f = 0; f < F; ++f) um[f] = s1[(e2 * F + f) * NP + vi];
+            for (int f = 0; f < F; ++f) um[f] = sst[(e2 * F + f) * NP + vi];
             if (MD && ((cnv[r].y >> 5) & 7) == 0 && ((cnv[r].y >> 8) & 15) != N) {
                 // the neighbour's trace of degree Nn brought to degree N (in the neighbour's
                 // face orientation), read at this node through the face permutation
@@ -334,7 +367,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             for (int f = 0; f < F; ++f) s2[((e2 * F + f) * 4 + phi) * NFP + m] = o[f];
         }
     }
-    __syncthreads();
+    if constexpr (Cfg::WAVEF) wave_lds_sync();
+    else __syncthreads();
 
     // where each face's chain lands on this thread's node: face f1..f4 -> (slab, index)
     const int l = N - i - j - k;
@@ -401,26 +435,35 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 }
             }
         }
-    } else
-    for (int t = tid; t < ne * 4 * NFP; t += NT) {
-        const int e2 = t / (4 * NFP), phi = (t / NFP) & 3, beta = t % NFP;
-        const int ab = dec[np3(N - 1) + beta];
-        const L0Pt<T> pt = l0_prep<N, T>(ab & 255, ab >> 8);
-        // all F evaluations (their LDS reads) before any store: the stores target another
-        // buffer, but the compiler cannot prove it and would serialise the fields
-        T v[F];
+    } else {
+        auto l0_item = [&](int e2, int phi, int beta) {
+            const int ab = dec[np3(N - 1) + beta];
+            const L0Pt<T> pt = l0_prep<N, T>(ab & 255, ab >> 8);
+            // all F evaluations (their LDS reads) before any store: the stores target another
+            // buffer, but the compiler cannot prove it and would serialise the fields
+            T v[F];
 #pragma unroll
-        for (int f = 0; f < F; ++f) v[f] = l0_eval<T>(s2 + ((e2 * F + f) * 4 + phi) * NFP, pt);
+            for (int f = 0; f < F; ++f) v[f] = l0_eval<T>(s2 + ((e2 * F + f) * 4 + phi) * NFP, pt);
 #pragma unroll
-        for (int f = 0; f < F; ++f) s1[((e2 * F + f) * 4 + phi) * QS + beta] = v[f];
+            for (int f = 0; f < F; ++f) s1[((e2 * F + f) * 4 + phi) * QS + beta] = v[f];
+        };
+        if constexpr (Cfg::WAVEF) {
+            for (int q = 0; q < CW; ++q) {
+                const int c = wave_u + NW * q;
+                if (c >= ne * 4) break;
+                for (int beta = tid & 63; beta < NFP; beta += 64) l0_item(c >> 2, c & 3, beta);
+            }
+        } else {
+            for (int t = tid; t < ne * 4 * NFP; t += NT) l0_item(t / (4 * NFP), (t / NFP) & 3, t % NFP);
+        }
     }
-    __syncthreads();
+    if constexpr (Cfg::WAVEF) wave_lds_sync();
+    else __syncthreads();
     // one reduction step of the chain, degree dd+1 -> dd, for every (element, face, node)
     auto chain_step = [&](const T *src, T *dst, int src_off, int dst_off, int dd) {
         const T inv = T(1) / T(dd + 1);
         const int nf = nfp2(dd);
-        for (int t = tid; t < ne * 4 * nf; t += NT) {
-            const int e2 = t / (4 * nf), phi = (t / nf) & 3, beta = t % nf;
+        auto item = [&](int e2, int phi, int beta) {
             const int ab = dec[np3(dd - 1) + beta];
             const int a = ab & 255, b = ab >> 8;
             const int idx = lin2(dd + 1, a, b);
@@ -438,6 +481,15 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 #pragma unroll
             for (int f = 0; f < F; ++f)
                 dst[((e2 * F + f) * 4 + phi) * QS + dst_off + beta] = fma(wa, x1[f], fma(wb, x2[f], wc * x0[f])) * inv;
+        };
+        if constexpr (Cfg::WAVEF) {
+            for (int q = 0; q < CW; ++q) {
+                const int c = wave_u + NW * q;
+                if (c >= ne * 4) break;
+                for (int beta = tid & 63; beta < nf; beta += 64) item(c >> 2, c & 3, beta);
+            }
+        } else {
+            for (int t = tid; t < ne * 4 * nf; t += NT) item(t / (4 * nf), (t / nf) & 3, t % nf);
         }
     };
 
@@ -445,8 +497,10 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 #pragma unroll
         for (int s = 1; s <= N; ++s) {
             chain_step(s1, s1, tet(N, s - 1), tet(N, s), N - s);
-            __syncthreads();
+            if constexpr (Cfg::WAVEF) wave_lds_sync();
+            else __syncthreads();
         }
+        if constexpr (Cfg::WAVEF) __syncthreads();  // every chain complete before the gather
         // ---- each node gathers l_slab * E from its four faces
         if (own) {
 #pragma unroll
@@ -517,7 +571,7 @@ int stage_launch_wave_md_mixed(int N, int tiling, const StageArgs &a, hipStream_
 constexpr int kStageMaxN = 10;
 // candidate tilings (elements per block, lift organisation), in autotuning order
 constexpr int kStageEB[] = {1, 2, 4};
-constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG, LIFT_GATHER_MFMA};
+constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG, LIFT_GATHER_MFMA, LIFT_WAVE};
 
 // tiling code: elements per block + 16 * lift organisation
 constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
@@ -548,6 +602,9 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             using G0 = std::integral_constant<int, LIFT_GATHER>;                                       \
             using P1 = std::integral_constant<int, LIFT_PINGPONG>;                                     \
             using M2 = std::integral_constant<int, LIFT_GATHER_MFMA>;                                  \
+            using W3 = std::integral_constant<int, LIFT_WAVE>;                                         \
+            one(std::integral_constant<int, 1>{}, W3{});                                               \
+            one(std::integral_constant<int, 2>{}, W3{});                                               \
             one(std::integral_constant<int, 1>{}, M2{});                                               \
             one(std::integral_constant<int, 2>{}, M2{});                                               \
             one(std::integral_constant<int, 4>{}, M2{});                                               \
@@ -585,12 +642,14 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             };                                                                                         \
             using G0 = std::integral_constant<int, LIFT_GATHER>;                                       \
             using P1 = std::integral_constant<int, LIFT_PINGPONG>;                                     \
+            using W3 = std::integral_constant<int, LIFT_WAVE>;                                         \
             one(std::integral_constant<int, 1>{}, G0{});                                               \
             one(std::integral_constant<int, 2>{}, G0{});                                               \
             one(std::integral_constant<int, 4>{}, G0{});                                               \
             one(std::integral_constant<int, 1>{}, P1{});                                               \
             one(std::integral_constant<int, 2>{}, P1{});                                               \
             one(std::integral_constant<int, 4>{}, P1{});                                               \
+            one(std::integral_constant<int, 1>{}, W3{});                                               \
             return rc;                                                                                 \
         });                                                                                            \
     }                                                                                                  \
