@@ -420,7 +420,7 @@ def main():
     # WRITE_SIZE passes over this workload (scaled by the 1 GiB copy calibration)
     traffic = None
     prof = os.path.join(ROOT, "profiles", "stage_maxwell_n9_k196608.json")
-    kname = f"k_stage<9, 1, true, {mx['tiling'] % 16}, {mx['tiling'] // 16}>"
+    kname = f"k_stage<9, 1, true, {mx['tiling'] % 16}, {mx['tiling'] // 16}, false>"
     if os.path.exists(prof):
         rec = json.load(open(prof))["kernels"].get(kname)
         if rec:
