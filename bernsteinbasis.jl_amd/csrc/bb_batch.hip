@@ -31,9 +31,12 @@ struct bb_batch {
     int64_t K = 0, words = 0;
     double sign[6] = {1, 1, 1, 1, 1, 1};
     std::vector<BatchGroup> groups;
-    double *geo = nullptr, *J = nullptr, *nJ = nullptr, *dte = nullptr, *tr = nullptr;
+    double *geo = nullptr, *J = nullptr, *nJ = nullptr, *dte = nullptr, *tr = nullptr, *evd = nullptr;
+    float *trf = nullptr, *evf = nullptr;
+    int32_t *erow = nullptr;
+    uint8_t *ecol = nullptr;
     int2 *conn = nullptr;
-    int64_t *nboff = nullptr, *troff = nullptr;
+    int64_t *nboff = nullptr, *troff = nullptr, *eoff = nullptr;
     double *q[2] = {nullptr, nullptr};
     int cur = 0;
     double *res64 = nullptr;
@@ -72,6 +75,12 @@ static StageArgs batch_args(bb_batch *b, const BatchGroup &g, int prec, int cur)
     a.tr = b->tr;
     a.troff = b->troff;
     a.dte = b->dte + g.start;
+    a.trf = b->trf;
+    a.erow = b->erow;
+    a.eoff = b->eoff;
+    a.ecol = b->ecol;
+    a.evd = b->evd;
+    a.evf = b->evf;
     return a;
 }
 
@@ -214,6 +223,29 @@ int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_
         g.count = e - g.start;
         b->groups.push_back(g);
     }
+    // elevations (Nn < N) as sparse rows; every table also in FP32
+    std::vector<int32_t> erow;
+    std::vector<int64_t> eoff(121, -1);
+    std::vector<uint8_t> ecol;
+    std::vector<double> evd;
+    for (int N = 1; N <= kStageMaxN; ++N)
+        for (int Nn = 1; Nn < N; ++Nn) {
+            const int64_t o = transfer_off_host[11 * N + Nn];
+            if (o < 0 || o + (int64_t)nfp2(N) * nfp2(Nn) > transfer_len) continue;
+            eoff[11 * N + Nn] = (int64_t)erow.size();
+            for (int r = 0; r < nfp2(N); ++r) {
+                erow.push_back((int32_t)evd.size());
+                for (int c = 0; c < nfp2(Nn); ++c) {
+                    const double v = transfer_host[o + (int64_t)r * nfp2(Nn) + c];
+                    if (v != 0.0) {
+                        ecol.push_back((uint8_t)c);
+                        evd.push_back(v);
+                    }
+                }
+            }
+            erow.push_back((int32_t)evd.size());
+        }
+    std::vector<float> evf(evd.begin(), evd.end()), trf(transfer_host, transfer_host + transfer_len);
     auto fail = [&](hipError_t err) {
         int r = bb_fail(ctx, BB_ERR_HIP, "batch allocation: %s", hipGetErrorString(err));
         bb_batch_destroy(b);
@@ -225,6 +257,11 @@ int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_
         (err = hipMalloc(&b->conn, K * 4 * sizeof(int2))) || (err = hipMalloc(&b->nboff, K * 4 * sizeof(int64_t))) ||
         (err = hipMalloc(&b->tr, (transfer_len > 0 ? transfer_len : 1) * sizeof(double))) ||
         (err = hipMalloc(&b->troff, 121 * sizeof(int64_t))) || (err = hipMalloc(&b->q[0], words * sizeof(double))) ||
+        (err = hipMalloc(&b->trf, (transfer_len > 0 ? transfer_len : 1) * sizeof(float))) ||
+        (err = hipMalloc(&b->erow, (erow.size() + 1) * sizeof(int32_t))) ||
+        (err = hipMalloc(&b->eoff, 121 * sizeof(int64_t))) ||
+        (err = hipMalloc(&b->ecol, ecol.size() + 1)) || (err = hipMalloc(&b->evd, (evd.size() + 1) * sizeof(double))) ||
+        (err = hipMalloc(&b->evf, (evf.size() + 1) * sizeof(float))) ||
         (err = hipMalloc(&b->q[1], words * sizeof(double))))
         return fail(err);
     if ((err = hipMemcpy(b->geo, geo_host, K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
@@ -234,7 +271,13 @@ int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_
         (err = hipMemcpy(b->conn, conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->nboff, nboff.data(), K * 4 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->tr, transfer_host, transfer_len * sizeof(double), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->troff, transfer_off_host, 121 * sizeof(int64_t), hipMemcpyHostToDevice)))
+        (err = hipMemcpy(b->troff, transfer_off_host, 121 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->trf, trf.data(), trf.size() * sizeof(float), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->erow, erow.data(), erow.size() * sizeof(int32_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->eoff, eoff.data(), 121 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->ecol, ecol.data(), ecol.size(), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->evd, evd.data(), evd.size() * sizeof(double), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->evf, evf.data(), evf.size() * sizeof(float), hipMemcpyHostToDevice)))
         return fail(err);
     for (auto &g : b->groups) {
         const std::vector<uint16_t> tab = stage_tables(g.N);
@@ -259,7 +302,8 @@ int bb_batch_destroy(bb_batch *b) {
     if (!b) return BB_ERR_NULL;
     hipStreamSynchronize(b->ctx->stream);
     batch_drop_graphs(b);
-    void *ptrs[] = {b->geo, b->J, b->nJ, b->dte, b->tr, b->conn, b->nboff, b->troff, b->q[0], b->q[1], b->res64, b->res32};
+    void *ptrs[] = {b->geo, b->J, b->nJ, b->dte, b->tr, b->conn, b->nboff, b->troff, b->q[0], b->q[1], b->res64, b->res32,
+                    b->trf, b->erow, b->eoff, b->ecol, b->evd, b->evf};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &g : b->groups) {

@@ -51,6 +51,15 @@ struct StageArgs {
     const double *tr;
     const int64_t *troff;
     const double *dte;
+    // the same matrices in FP32 (mixed precision), and the elevations (Nn < N) as sparse
+    // rows: row m of pair (N, Nn) holds entries erow[eoff[11 N + Nn] + m] ..
+    // erow[eoff[11 N + Nn] + m + 1] - 1 of ecol (neighbour face node) and evd / evf (weight)
+    const float *trf;
+    const int32_t *erow;
+    const int64_t *eoff;
+    const uint8_t *ecol;
+    const double *evd;
+    const float *evf;
 };
 
 // Five-stage fourth-order 2N-storage RK (Carpenter & Kennedy, NASA TM-109112, 1994)
@@ -342,15 +351,29 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int a = ab & 255, b = ab >> 8, c = N - a - b;
                 const int q0 = perm_entry(p, 0), q1 = perm_entry(p, 1);
                 const int a2 = q0 == 0 ? a : (q0 == 1 ? b : c), b2 = q1 == 0 ? a : (q1 == 1 ? b : c);
-                const double *Tr = A.tr + A.troff[11 * N + Nn] + (int64_t)lin2(N, a2, b2) * nfn;
+                const int mp = lin2(N, a2, b2);
                 const T *tn = s3 + (e2 * 4 + phi) * F * kTraceMax;
                 T up[F];
 #pragma unroll
                 for (int f = 0; f < F; ++f) up[f] = T(0);
-                for (int jn = 0; jn < nfn; ++jn) {
-                    const T w = T(Tr[jn]);
+                if (Nn < N) {
+                    // elevation: the few neighbour nodes below this one (sparse row)
+                    const int32_t *rp = A.erow + A.eoff[11 * N + Nn] + mp;
+                    const int j1 = rp[1];
+                    for (int jj = rp[0]; jj < j1; ++jj) {
+                        const T w = MIXED ? T(A.evf[jj]) : T(A.evd[jj]);
+                        const int jn = A.ecol[jj];
 #pragma unroll
-                    for (int f = 0; f < F; ++f) up[f] += w * tn[f * kTraceMax + jn];
+                        for (int f = 0; f < F; ++f) up[f] += w * tn[f * kTraceMax + jn];
+                    }
+                } else {
+                    // L2 projection from a higher degree: a dense row
+                    const int64_t ro = A.troff[11 * N + Nn] + (int64_t)mp * nfn;
+                    for (int jn = 0; jn < nfn; ++jn) {
+                        const T w = MIXED ? T(A.trf[ro + jn]) : T(A.tr[ro + jn]);
+#pragma unroll
+                        for (int f = 0; f < F; ++f) up[f] += w * tn[f * kTraceMax + jn];
+                    }
                 }
 #pragma unroll
                 for (int f = 0; f < F; ++f) d[f] = up[f] - um[f];
