@@ -238,6 +238,45 @@ def lift_study(ctx, K=65536):
     return rows
 
 
+def wave_n7_bench(ctx, steps=20, warmup=3, n=32, plain_steps=3):
+    """Secondary workload at the low end of the metric's degree range: the 3-D acoustic
+    wave (p,u,v,w) at N=7 (Np=120) on the same 32^3 x 6 jittered Kuhn mesh (seed 3),
+    periodic, Gaussian pulse p = exp(-25 |x - 0.5|^2), LSRK45 dt = 0.5 h / N^2; fused mixed,
+    fused FP64 and plain FP64 steps, each autotuned and replayed from a hipGraph."""
+    import bbdg
+    from bbdg import basis, mesh
+
+    N = 7
+    m = mesh.kuhn_mesh(n, 0.1, 3, periodic=True)
+    q0 = basis.interpolate(m, N, lambda x, y, z: np.stack(
+        [np.exp(-25 * ((x - .5) ** 2 + (y - .5) ** 2 + (z - .5) ** 2)), 0 * x, 0 * x, 0 * x]))
+    s = bbdg.Solver(m, N, "wave", ctx=ctx)
+    dt = 0.5 * m.h / N ** 2
+    out = {"workload": f"wave3d N=7 K={m.K} (32^3x6 Kuhn, jitter seed 3) periodic upwind LSRK45", "dofs": s.dofs}
+    states = {}
+    for prec in ("mixed", "f64"):
+        s.set_state(q0)
+        t = s.autotune(prec)
+        s.set_state(q0)
+        ms = time_graph(s, dt, steps, warmup, prec)
+        states[prec] = s.get_state()
+        out[f"fused_{prec}"] = {"ms_per_step": ms, "value": s.dofs / (ms * 1e-3), "tiling": t}
+        log(f"wave N=7 fused {prec}: tiling={t} {ms:.3f} ms/step {s.dofs / ms * 1e3:.3e} DOF/s")
+    s.set_state(q0)
+    s.step(dt, 1, "plain", "f64")
+    s.ctx.sync()
+    s.ctx.timer_start()
+    s.step(dt, plain_steps, "plain", "f64")
+    ms = s.ctx.timer_stop() / plain_steps
+    out["plain_f64"] = {"ms_per_step": ms, "value": s.dofs / (ms * 1e-3)}
+    log(f"wave N=7 plain f64: {ms:.3f} ms/step")
+    out["speedup_mixed_vs_plain_f64"] = out["fused_mixed"]["value"] / out["plain_f64"]["value"]
+    out["rel_l2_mixed_vs_f64"] = float(np.linalg.norm(states["mixed"] - states["f64"]) / np.linalg.norm(states["f64"]))
+    out["steps_compared"] = steps + warmup
+    s.close()
+    return out
+
+
 def batch_bench(ctx, steps=100, n_inst=64, seed=5):
     """Batching workload: n_inst independent acoustic-wave instances, each a jittered
     periodic Kuhn mesh of n^3 x 6 tets with n ~ U{4..16}; every element has its own
@@ -336,6 +375,7 @@ def main():
                     help="run only the lift conditioning study (single-face lift precision variants)")
     ap.add_argument("--ops-only", action="store_true", help="run only the operator microbenchmark")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-degree workload")
+    ap.add_argument("--no-wave7", action="store_true", help="skip the N=7 wave workload")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched mixed-degree workload")
     ap.add_argument("--batch-steps", type=int, default=100, help="timed steps of the batched workload")
     ap.add_argument("--stage-only", action="store_true",
@@ -433,6 +473,7 @@ def main():
         ops = operator_bench(bbdg.default_context())
         for k, v in ops.items():
             log(f"operator {k}: {v['us']:.1f} us, {v['GBs']:.0f} GB/s ({v['frac_of_8TBs']:.2f} of 8 TB/s)")
+    wave7 = wave_n7_bench(bbdg.default_context()) if not args.no_wave7 else None
     batch = None
     if not args.no_batch:
         batch = batch_bench(bbdg.default_context(), steps=args.batch_steps)
@@ -477,6 +518,7 @@ def main():
                      "stream_copy_GBs": copy_gbs, "frac_of_stream_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
         "operators_N7_K262144_F4": ops,
+        "wave_n7": wave7,
         "batch_mixed_degree": batch,
     }
     print(json.dumps(out))
