@@ -89,14 +89,6 @@ constexpr int kTraceMax = 66;  // face nodes of the highest degree of a fused ke
 //             buffer, since fast waves overwrite chain slabs while slow ones still read it.
 enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1, LIFT_GATHER_MFMA = 2, LIFT_WAVE = 3 };
 
-// LDS written by one lane and read by another lane of the same wave: a wave executes its
-// LDS instructions in order; the fences keep the compiler from moving accesses across.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = false> struct StageCfg {
     using T = std::conditional_t<MIXED, float, double>;

@@ -6,6 +6,15 @@
 
 namespace bb {
 
+// LDS written by one lane and read by another lane of the same wave: a wave executes its
+// LDS instructions in order; the fences keep the compiler from moving accesses across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // linear index -> (i,j,k) for degree N (k outer, j, i inner)
 template <int N> __device__ __forceinline__ void decode3(int n, int &i, int &j, int &k) {
     int kk = 0;
