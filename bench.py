@@ -277,6 +277,90 @@ def wave_n7_bench(ctx, steps=20, warmup=3, n=32, plain_steps=3):
     return out
 
 
+def nodal_bench(ctx, Ns=(3, 5, 7, 9), n=32, reps=5):
+    """Dense nodal comparator (extras/3d_nodal_wave.jl:57-73) against the Bernstein paths,
+    3-D acoustic wave on the 32^3 x 6 periodic mesh (seed 3), FP64, per degree:
+    - one RHS: dense nodal (nine Np x Np and one Np x 4Nfp library GEMMs over all K*F
+      columns, no RK update) against the Bernstein plain path (one kernel per operator) and
+      the fused stage kernel (FP64 and mixed), both timed per LSRK stage (RHS + RK update);
+    - the README's volume / surface split: nine dense derivative GEMMs against nine
+      Bernstein derivative stencils (bb_deriv_apply), one dense LIFT GEMM against the
+      Bernstein 4-face lift (bb_lift_apply), on the same K*F columns."""
+    import bbdg
+    from bbdg import basis, mesh
+    from bbdg.nodal import NodalOperators, NodalRHS
+    from bbdg.operators import (BernsteinDerivativeMatrix_3D_r, BernsteinDerivativeMatrix_3D_s,
+                                BernsteinDerivativeMatrix_3D_t)
+
+    m = mesh.kuhn_mesh(n, 0.1, 3, periodic=True)
+    fn = lambda x, y, z: np.stack([np.exp(-25 * ((x - .5) ** 2 + (y - .5) ** 2 + (z - .5) ** 2)), 0 * x, 0 * x, 0 * x])
+
+    def timed(f):
+        f()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(reps):
+            f()
+        return ctx.timer_stop() / reps
+
+    rows = []
+    F, K = 4, m.K
+    for N in Ns:
+        NP, NFP = (N + 1) * (N + 2) * (N + 3) // 6, (N + 1) * (N + 2) // 2
+        ops = NodalOperators(N, ctx)
+        c = basis.interpolate(m, N, fn)
+        qn = bbdg.DeviceArray.from_numpy(np.ascontiguousarray(np.einsum("ab,kbf->kfa", ops.V, c)), ctx=ctx)
+        nr = NodalRHS(m, N, "wave", ops=ops, ctx=ctx)
+        ms_dense = timed(lambda: nr.rhs_device(qn))
+        ms_dense_vol = timed(nr.derivatives_device)
+        ms_dense_surf = timed(nr.lift_device)
+        flops = ops.flops_per_rhs(K, F)
+        # Bernstein operators on the same columns: nine derivative stencils, one 4-face lift
+        sh, st = (NP, K, F), (1, F * NP, NP)
+        Ds = [cls(N) for cls in (BernsteinDerivativeMatrix_3D_r, BernsteinDerivativeMatrix_3D_s,
+                                 BernsteinDerivativeMatrix_3D_t)]
+
+        def bern_volume():
+            for cc in range(3):
+                for ax in range(3):
+                    bbdg.mul_(nr.d9[3 * cc + ax].view(sh, st), Ds[ax], nr.fu[cc].view(sh, st))
+
+        ms_bern_vol = timed(bern_volume)
+        Lb = bbdg.BernsteinLift(N)
+        fv = nr.flux.view((4 * NFP, K, F), (1, F * 4 * NFP, 4 * NFP))
+        ms_bern_surf = timed(lambda: bbdg.mul_(nr.du.view(sh, st), Lb, fv))
+        nr.close()
+        qn.free()
+        s = bbdg.Solver(m, N, "wave", ctx=ctx)
+        s.set_state(c)
+        dt = 0.5 * m.h / N ** 2
+        s.step(dt, 1, "plain", "f64")
+        ctx.sync()
+        ctx.timer_start()
+        s.step(dt, 1, "plain", "f64")
+        ms_plain = ctx.timer_stop() / 5  # one step = 5 stages
+        fused = {}
+        for prec in ("f64", "mixed"):
+            s.autotune(prec)
+            fused[prec] = time_stage_kernel(s, prec, reps=4)
+        s.close()
+        row = {"N": N, "dofs": K * F * NP, "ms_dense_nodal_rhs_f64": ms_dense,
+               "dense_gemm_tflops": flops / (ms_dense * 1e-3) * 1e-12,
+               "ms_dense_volume": ms_dense_vol, "ms_dense_surface": ms_dense_surf,
+               "ms_bernstein_volume": ms_bern_vol, "ms_bernstein_surface": ms_bern_surf,
+               "speedup_volume": ms_dense_vol / ms_bern_vol, "speedup_surface": ms_dense_surf / ms_bern_surf,
+               "ms_bernstein_plain_stage_f64": ms_plain, "ms_bernstein_fused_stage_f64": fused["f64"],
+               "ms_bernstein_fused_stage_mixed": fused["mixed"],
+               "speedup_plain_f64_vs_dense": ms_dense / ms_plain,
+               "speedup_fused_f64_vs_dense": ms_dense / fused["f64"]}
+        rows.append(row)
+        log(f"nodal N={N}: dense RHS {ms_dense:.3f} ms ({row['dense_gemm_tflops']:.1f} TFLOP/s; volume "
+            f"{ms_dense_vol:.3f}, surface {ms_dense_surf:.3f}); Bernstein volume {ms_bern_vol:.3f}, surface "
+            f"{ms_bern_surf:.3f}; stages: plain {ms_plain:.3f}, fused f64 {fused['f64']:.3f}, fused mixed "
+            f"{fused['mixed']:.3f} ms")
+    return rows
+
+
 def batch_bench(ctx, steps=100, n_inst=64, seed=5):
     """Batching workload: n_inst independent acoustic-wave instances, each a jittered
     periodic Kuhn mesh of n^3 x 6 tets with n ~ U{4..16}; every element has its own
@@ -376,6 +460,8 @@ def main():
     ap.add_argument("--ops-only", action="store_true", help="run only the operator microbenchmark")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-degree workload")
     ap.add_argument("--no-wave7", action="store_true", help="skip the N=7 wave workload")
+    ap.add_argument("--no-nodal", action="store_true", help="skip the dense nodal comparator")
+    ap.add_argument("--nodal-only", action="store_true", help="run only the dense nodal comparator")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched mixed-degree workload")
     ap.add_argument("--batch-steps", type=int, default=100, help="timed steps of the batched workload")
     ap.add_argument("--stage-only", action="store_true",
@@ -391,6 +477,9 @@ def main():
 
     if args.lift_study:
         print(json.dumps({"lift_conditioning": lift_study(bbdg.default_context())}))
+        return
+    if args.nodal_only:
+        print(json.dumps({"dense_nodal_comparator": nodal_bench(bbdg.default_context())}))
         return
     if args.batch_only:
         ctx = bbdg.default_context()
@@ -474,6 +563,7 @@ def main():
         for k, v in ops.items():
             log(f"operator {k}: {v['us']:.1f} us, {v['GBs']:.0f} GB/s ({v['frac_of_8TBs']:.2f} of 8 TB/s)")
     wave7 = wave_n7_bench(bbdg.default_context()) if not args.no_wave7 else None
+    nodal = nodal_bench(bbdg.default_context()) if not args.no_nodal else None
     batch = None
     if not args.no_batch:
         batch = batch_bench(bbdg.default_context(), steps=args.batch_steps)
@@ -519,6 +609,7 @@ def main():
         "cpu_baseline": cpu,
         "operators_N7_K262144_F4": ops,
         "wave_n7": wave7,
+        "dense_nodal_comparator": nodal,
         "batch_mixed_degree": batch,
     }
     print(json.dumps(out))

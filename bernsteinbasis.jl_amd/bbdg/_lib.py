@@ -78,6 +78,9 @@ SIGS = {
     "bb_solver_autotune": (i32, [vp, i32, C.POINTER(C.c_int)]),
     "bb_solver_set_tuning": (i32, [vp, i32, i32]),
     "bb_solver_stage_bytes": (i32, [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bb_gemm": (i32, [vp, i32, i64, i64, i64, dbl, vp, i64, vp, i64, dbl, vp, i64]),
+    "bb_pde_flux_dirs": (i32, [vp, i32, i32, i64, vp, vp, vp, vp]),
+    "bb_rhs_combine": (i32, [vp, i32, i32, i64, vp, vp, vp, vp]),
     "bb_batch_create": (i32, [vp, i32, i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64, C.POINTER(vp)]),
     "bb_batch_destroy": (i32, [vp]),
     "bb_batch_state_words": (i32, [vp, C.POINTER(C.c_int64)]),

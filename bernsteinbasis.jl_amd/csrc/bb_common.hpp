@@ -66,11 +66,14 @@ struct bb_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool capturing = false;
+    void *blas = nullptr;  // library GEMM handle, created on first use (bb_gemm)
     std::string err;
 };
 
 // error helpers (host)
 int bb_fail(bb_ctx *ctx, int code, const char *fmt, ...);
+// releases the context's library GEMM handle (bb_dense.hip)
+void bb_blas_release(bb_ctx *ctx);
 #define BB_HIP(ctx, call)                                                                           \
     do {                                                                                            \
         hipError_t e_ = (call);                                                                     \

@@ -61,6 +61,7 @@ int bb_ctx_destroy(bb_ctx *ctx) {
     if (!ctx) return BB_ERR_NULL;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    bb_blas_release(ctx);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
     hipStreamDestroy(ctx->stream);

@@ -197,6 +197,21 @@ int bb_solver_set_tuning(bb_solver *s, int prec, int tiling);
  * in/out, neighbour traces, geometry, connectivity) and the DOF count K*Np*F */
 int bb_solver_stage_bytes(bb_solver *s, int prec, double *bytes, double *dofs);
 
+/* ---- dense nodal comparator (extras/3d_nodal_wave.jl:57-73) ------------------------
+ * The reference's comparison RHS applies dense Np x Np derivative matrices and a dense
+ * Np x 4Nfp lift to nodal values (mul!(dfxdr, Dr, fxu), ..., mul!(du, LIFT, flux, 1, -1)).
+ * bb_gemm is that product over all columns of all elements: C = alpha A B + beta C,
+ * column-major, no transposes, on the context's stream (vendor BLAS, FP64 or FP32).
+ * bb_pde_flux_dirs computes fx.(q), fy.(q), fz.(q) (examples/3d_wave.jl:59-61) on
+ * [K][F][Np]; bb_rhs_combine does du -= (wave) or += (Maxwell) the nine geometric
+ * products of d[0..8] = (dfx/dr, dfx/ds, dfx/dt, dfy/dr, ..., dfz/dt), then du ./= J
+ * (examples/3d_wave.jl:75-83), all [K][F][Np] FP64. */
+int bb_gemm(bb_ctx *ctx, int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void *A, int64_t lda,
+            const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
+int bb_pde_flux_dirs(bb_ctx *ctx, int pde, int N, int64_t K, const double *q, double *fx, double *fy, double *fz);
+int bb_rhs_combine(bb_ctx *ctx, int pde, int N, int64_t K, double *du, const double *const d[9], const double *geo,
+                   const double *J);
+
 /* ---- batches of elements of mixed degree ------------------------------------------
  * No single counterpart in the reference, whose examples run one mesh at one degree
  * (examples/3d_wave.jl:36-134); this is that RHS + LSRK45 for many independent meshes
