@@ -11,9 +11,9 @@
 //   one is written), res [K][F][Np] (FP64, or FP32 in mixed precision), geo [K][9],
 //   J [K], nJ [K][4][3], conn [K][4] = {neighbour, face | perm << 2 | bc << 5}.
 //
-// LDS (T = FP64, or FP32 in mixed precision):
+// LDS (T = FP64, or FP32 in mixed precision), every array as node records of F values:
 //   s1: the element's coefficients, later chain buffer A
-//   s2: interface flux (4 faces x Nfp per field), later chain buffer B
+//   s2: interface flux (4 faces x Nfp records), later chain buffer B
 // Each thread owns one volume node: its F volume terms and F lift accumulators stay in
 // registers (accumulators always FP64 -- the signed binomially weighted sum of the four
 // faces' chains, where cancellation happens), so no LDS scatter and no atomics.
@@ -120,8 +120,8 @@ __host__ __device__ constexpr int l0_khi(int N, int mt) {
     return ((16 * mt + 15 + (N + 1)) < nfp2(N) - 1 ? (16 * mt + 15 + (N + 1)) : nfp2(N) - 1) / 4;
 }
 
-// L0 = (N+1)^2/2 (E_N^{N+1})^T E_N^{N+1} as a dense Nfp x Nfp matrix (the 7-point stencil
-// of l0_apply; every entry is a multiple of 1/2, exact in FP32), laid out as the A operand
+// 2 L0 = (N+1)^2 (E_N^{N+1})^T E_N^{N+1} as a dense Nfp x Nfp matrix (the 7-point stencil
+// of l0_apply without its factor 1/2: integer entries, exact in FP32), laid out as the A operand
 // of v_mfma_f32_16x16x4f32: for row tile mt, lane L and k step kk the value
 // L0[16 mt + L % 16][4 kk + L / 16] (zero outside the matrix), stored [mt][L][kk].
 inline std::vector<float> l0_mfma_table(int N) {
@@ -134,7 +134,7 @@ inline std::vector<float> l0_mfma_table(int N) {
                                   {a + 1, b - 1, (a + 1) * b}, {a + 1, b, (a + 1) * c}, {a - 1, b + 1, (b + 1) * a},
                                   {a, b + 1, (b + 1) * c},     {a - 1, b, (c + 1) * a}, {a, b - 1, (c + 1) * b}};
             for (auto &q : nb)
-                if (q[2] != 0) L0[(size_t)r * NFP + lin2(N, q[0], q[1])] += 0.5 * q[2];
+                if (q[2] != 0) L0[(size_t)r * NFP + lin2(N, q[0], q[1])] += q[2];
         }
     std::vector<float> t((size_t)MT * 64 * KT, 0.f);
     bool outside = false;
@@ -148,6 +148,16 @@ inline std::vector<float> l0_mfma_table(int N) {
             }
     if (outside) t.clear();  // an entry the kernel would skip: refuse the table
     return t;
+}
+
+// Weight of slab s of the unnormalised lift chain of the stage kernel: slab 0 is 2 L0 x
+// and step t (degree N-t+1 -> N-t) omits the 1/(N-t+1) of reduction_multiply!, so slab s
+// is 2 N!/(N-s)! times the slab of face_lift_multiply! (src/BernsteinLift.jl:139-179);
+// its l_s (src/constants.jl:10-12) divided by that integer, rounded once
+__host__ __device__ constexpr double lw_chain(int N, int s) {
+    double d = 2.0;
+    for (int t = 0; t < s; ++t) d *= (N - t);
+    return lj(N, s) / d;
 }
 
 // Index tables of degree N for the stage kernel, built once on the host (uint16 each):
@@ -191,14 +201,15 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     using T = typename Cfg::T;
     using P = Pde<PDE>;
     constexpr int F = Cfg::F, NP = Cfg::NP, NFP = Cfg::NFP, NT = Cfg::NT;
-    __shared__ T s1[Cfg::R1];
-    __shared__ T s2[Cfg::R2];
-    __shared__ T s0[Cfg::WAVEF ? Cfg::R0 : 1];
-    T *sst = Cfg::WAVEF ? s0 : s1;  // the element state in LDS
-    __shared__ T s3[MD ? Cfg::R3 : 1];
+    // every LDS array holds node records of F values (field fastest): see recv_load
+    __shared__ __align__(16) T s1[Cfg::R1];
+    __shared__ __align__(16) T s2[Cfg::R2];
+    __shared__ __align__(16) T s0[Cfg::WAVEF ? Cfg::R0 : 1];
+    T *sst = Cfg::WAVEF ? s0 : s1;  // the element state in LDS: [element][node][field]
+    __shared__ __align__(16) T s3[MD ? Cfg::R3 : 1];
     __shared__ uint4 tab16[Cfg::TAB16];
     const uint16_t *tab = reinterpret_cast<const uint16_t *>(tab16);
-    __shared__ double lw[N + 1];  // l_0 = 1, l_1 .. l_N (src/constants.jl:10-12)
+    __shared__ double lw[N + 1];  // slab weights of the unnormalised chain (lw_chain)
     const uint16_t *ijk = tab, *dec = tab + NP, *fvid = tab + 2 * NP, *nvid = tab + 2 * NP + 4 * NFP;
 
     const int tid = threadIdx.x;
@@ -221,7 +232,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     if (tid <= N) {
 #pragma unroll
         for (int s = 0; s <= N; ++s)
-            if (tid == s) lw[s] = lj(N, s);
+            if (tid == s) lw[s] = lw_chain(N, s);
     }
     // matrix-core L0: this wave's first row tile of L0 as A operands, needed after the flux
     constexpr int MT = Cfg::MT, KT = Cfg::KT, NW = NT / 64;
@@ -284,7 +295,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         }
     }
     if constexpr (MD) {
-        // whole face traces of neighbours of another degree -> s3[(element, face, field)][node]
+        // whole face traces of neighbours of another degree -> s3[(element, face)][node][field]
         for (int t = tid; t < ne * 4 * kTraceMax; t += NT) {
             const int e2 = t / (4 * kTraceMax), phi = (t / kTraceMax) & 3, jn = t % kTraceMax;
             const int2 cn = A.conn[(e0 + e2) * 4 + phi];
@@ -294,14 +305,18 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 decode2(Nn, jn, a, b);
                 const int npn = np3(Nn);
                 const double *qn = A.q_nb + A.nboff[(e0 + e2) * 4 + phi] + face_vidx(Nn, cn.y & 3, a, b);
+                T v[F];
 #pragma unroll
-                for (int f = 0; f < F; ++f) s3[((e2 * 4 + phi) * F + f) * kTraceMax + jn] = T(qn[f * npn]);
+                for (int f = 0; f < F; ++f) v[f] = T(qn[f * npn]);
+                recv_store<F>(s3 + ((e2 * 4 + phi) * kTraceMax + jn) * F, v);
             }
         }
     }
     if (own) {
+        T v[F];
 #pragma unroll
-        for (int f = 0; f < F; ++f) sst[(el * F + f) * NP + n] = T(qv[f]);
+        for (int f = 0; f < F; ++f) v[f] = T(qv[f]);
+        recv_store<F>(sst + (el * NP + n) * F, v);
     }
     __syncthreads();  // state visible
 
@@ -312,17 +327,28 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     const int pk = ijk[n];
     const int i = pk & 15, j = (pk >> 4) & 15, k = pk >> 8;
     if (own) {
+        // halved geometric factors: grad2_rec returns twice the reference derivatives
         const double *g = A.geo + eg * 9;
-        const T g0 = T(g[0]), g1 = T(g[1]), g2 = T(g[2]), g3 = T(g[3]), g4 = T(g[4]), g5 = T(g[5]), g6 = T(g[6]),
-                g7 = T(g[7]), g8 = T(g[8]);
+        T gh[9];
 #pragma unroll
-        for (int f = 0; f < F; ++f) {
-            T dr, ds, dt;
-            grad_ref<N, T>(sst + (el * F + f) * NP, n, i, j, k, dr, ds, dt);
-            const T gx = g0 * dr + g1 * ds + g2 * dt;
-            const T gy = g3 * dr + g4 * ds + g5 * dt;
-            const T gz = g6 * dr + g7 * ds + g8 * dt;
-            P::template vol_add<T>(f, gx, gy, gz, vol);
+        for (int c = 0; c < 9; ++c) gh[c] = T(0.5 * g[c]);
+        // fields in groups of G (one 16- or 8-byte word per stencil point and group)
+        constexpr int G = (sizeof(T) == 4 && F % 4 == 0) ? 4 : 2;
+        const T *xs = sst + el * NP * F;
+#pragma unroll
+        for (int g0 = 0; g0 < F; g0 += G) {
+            T dr[G], ds[G], dt[G];
+            grad2_rec<N, F, G, T>(xs + g0, n, i, j, k, dr, ds, dt);
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                const T gx = gh[0] * dr[q] + gh[1] * ds[q] + gh[2] * dt[q];
+                const T gy = gh[3] * dr[q] + gh[4] * ds[q] + gh[5] * dt[q];
+                const T gz = gh[6] * dr[q] + gh[7] * ds[q] + gh[8] * dt[q];
+                P::template vol_add<T>(g0 + q, gx, gy, gz, vol);
+            }
+            // one group at a time: hoisting the next group's reads above this group's
+            // arithmetic would hold all 13 F values per node in registers at once
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 
@@ -333,8 +359,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         if (fitem(r, e2, phi, m)) {
             const int vi = fvid[phi * NFP + m];
             T um[F], d[F], o[F];
-#pragma unroll
-            for (int f = 0; f < F; ++f) um[f] = sst[(e2 * F + f) * NP + vi];
+            recv_load<F>(sst + (e2 * NP + vi) * F, um);
             if (MD && ((cnv[r].y >> 5) & 7) == 0 && ((cnv[r].y >> 8) & 15) != N) {
                 // the neighbour's trace of degree Nn brought to degree N (in the neighbour's
                 // face orientation), read at this node through the face permutation
@@ -344,8 +369,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int q0 = perm_entry(p, 0), q1 = perm_entry(p, 1);
                 const int a2 = q0 == 0 ? a : (q0 == 1 ? b : c), b2 = q1 == 0 ? a : (q1 == 1 ? b : c);
                 const int mp = lin2(N, a2, b2);
-                const T *tn = s3 + (e2 * 4 + phi) * F * kTraceMax;
-                T up[F];
+                const T *tn = s3 + (e2 * 4 + phi) * kTraceMax * F;
+                T up[F], tv[F];
 #pragma unroll
                 for (int f = 0; f < F; ++f) up[f] = T(0);
                 if (Nn < N) {
@@ -354,17 +379,18 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                     const int j1 = rp[1];
                     for (int jj = rp[0]; jj < j1; ++jj) {
                         const T w = MIXED ? T(A.evf[jj]) : T(A.evd[jj]);
-                        const int jn = A.ecol[jj];
+                        recv_load<F>(tn + A.ecol[jj] * F, tv);
 #pragma unroll
-                        for (int f = 0; f < F; ++f) up[f] += w * tn[f * kTraceMax + jn];
+                        for (int f = 0; f < F; ++f) up[f] += w * tv[f];
                     }
                 } else {
                     // L2 projection from a higher degree: a dense row
                     const int64_t ro = A.troff[11 * N + Nn] + (int64_t)mp * nfn;
                     for (int jn = 0; jn < nfn; ++jn) {
                         const T w = MIXED ? T(A.trf[ro + jn]) : T(A.tr[ro + jn]);
+                        recv_load<F>(tn + jn * F, tv);
 #pragma unroll
-                        for (int f = 0; f < F; ++f) up[f] += w * tn[f * kTraceMax + jn];
+                        for (int f = 0; f < F; ++f) up[f] += w * tv[f];
                     }
                 }
 #pragma unroll
@@ -378,8 +404,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             }
             const double *nj = A.nJ + ((e0 + e2) * 4 + phi) * 5;
             P::template flux<T>(A.flux_kind, d, T(nj[0]), T(nj[1]), T(nj[2]), T(nj[3]), T(nj[4]), o);
-#pragma unroll
-            for (int f = 0; f < F; ++f) s2[((e2 * F + f) * 4 + phi) * NFP + m] = o[f];
+            recv_store<F>(s2 + ((e2 * 4 + phi) * NFP + m) * F, o);
         }
     }
     if constexpr (Cfg::WAVEF) wave_lds_sync();
@@ -394,14 +419,14 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     for (int f = 0; f < F; ++f) acc[f] = 0.0;
     const int order[4] = {3, 0, 1, 2};  // face 4 first, then G1, G2, G3 (BernsteinLift.jl:202-217)
 
-    // chain storage: GATHER keeps every slab of every chain ([chain][Np], slab s at the
-    // volume offset tet(N,s)); PINGPONG keeps two [chain][Nfp] buffers.  Work items are
-    // (element, face, face node); the index and weight arithmetic is done once per item and
-    // reused for the F fields of the item.
+    // chain storage, chain c = 4 element + face: GATHER keeps every slab of every chain
+    // ([chain][Np][field], slab s at the volume offset tet(N,s)); PINGPONG keeps two
+    // [chain][Nfp][field] buffers.  The chain is unnormalised: slab 0 is 2 L0 x and each
+    // step omits its 1/(d+1); the slab weights lw carry both factors.
     constexpr int QS = Cfg::GATHER ? NP : NFP;
     // ---- L0 on every (element, face): flux s2 -> slab 0 (buffer s1)
     if constexpr (LM == LIFT_GATHER_MFMA) {
-        // chain column col = (element, field, face) is s2[col * Nfp ..] and s1[col * Np ..].
+        // chain column col = F (4 element + face) + field reads s2 and writes s1 at stride F.
         // Wave w computes the row tiles mt = w, w + NW, .. for all column tiles at once:
         // the B operands of every column tile are read first, then the k steps advance
         // the column tiles' independent accumulators in turn.
@@ -418,14 +443,14 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             float b[NTLM][KT];
 #pragma unroll
             for (int nt = 0; nt < NTLM; ++nt) {
-                const int col = nt * 16 + (lane & 15);
-                const T *xb = s2 + (col < ncols ? col : ncols - 1) * NFP;
+                const int col = min(nt * 16 + (lane & 15), ncols - 1);
+                const T *xb = s2 + (col / F) * NFP * F + col % F;
 #pragma unroll
                 for (int kk = 0; kk < KT; ++kk) {
                     if (kk < klo || kk > khi) continue;
                     const int k = 4 * kk + (lane >> 4);
                     // rows of the padded k range meet zero columns of L0, but must be finite
-                    b[nt][kk] = (4 * kk + 3 < NFP || k < NFP) ? xb[k < NFP ? k : NFP - 1] : 0.f;
+                    b[nt][kk] = (4 * kk + 3 < NFP || k < NFP) ? xb[(k < NFP ? k : NFP - 1) * F] : 0.f;
                 }
             }
             v4f acc[NTLM];
@@ -442,110 +467,165 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             for (int nt = 0; nt < NTLM; ++nt) {
                 const int col = nt * 16 + (lane & 15);
                 if (col < ncols) {
+                    T *yb = s1 + (col / F) * QS * F + col % F;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int beta = 16 * mt + 4 * (lane >> 4) + r;
-                        if (beta < NFP) s1[col * QS + beta] = acc[nt][r];
+                        if (beta < NFP) yb[beta * F] = acc[nt][r];
                     }
                 }
             }
         }
     } else {
-        auto l0_item = [&](int e2, int phi, int beta) {
+        auto l0_item = [&](int c, int beta) {
             const int ab = dec[np3(N - 1) + beta];
             const L0Pt<T> pt = l0_prep<N, T>(ab & 255, ab >> 8);
-            // all F evaluations (their LDS reads) before any store: the stores target another
-            // buffer, but the compiler cannot prove it and would serialise the fields
-            T v[F];
+            const T *x = s2 + c * NFP * F;
+            T v[F], y[F];
+            recv_load<F>(x + F * pt.i0, y);
 #pragma unroll
-            for (int f = 0; f < F; ++f) v[f] = l0_eval<T>(s2 + ((e2 * F + f) * 4 + phi) * NFP, pt);
+            for (int f = 0; f < F; ++f) v[f] = pt.w0 * y[f];
+            recv_load<F>(x + F * pt.i1, y);
 #pragma unroll
-            for (int f = 0; f < F; ++f) s1[((e2 * F + f) * 4 + phi) * QS + beta] = v[f];
+            for (int f = 0; f < F; ++f) v[f] += pt.w1 * y[f];
+            recv_load<F>(x + F * pt.i2, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] += pt.w2 * y[f];
+            recv_load<F>(x + F * pt.i3, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] += pt.w3 * y[f];
+            recv_load<F>(x + F * pt.i4, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] += pt.w4 * y[f];
+            recv_load<F>(x + F * pt.i5, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] += pt.w5 * y[f];
+            recv_load<F>(x + F * pt.i6, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] += pt.w6 * y[f];
+            recv_store<F>(s1 + (c * QS + beta) * F, v);
         };
         if constexpr (Cfg::WAVEF) {
             for (int q = 0; q < CW; ++q) {
                 const int c = wave_u + NW * q;
                 if (c >= ne * 4) break;
-                for (int beta = tid & 63; beta < NFP; beta += 64) l0_item(c >> 2, c & 3, beta);
+                for (int beta = tid & 63; beta < NFP; beta += 64) l0_item(c, beta);
             }
         } else {
-            for (int t = tid; t < ne * 4 * NFP; t += NT) l0_item(t / (4 * NFP), (t / NFP) & 3, t % NFP);
+            for (int t = tid; t < ne * 4 * NFP; t += NT) l0_item(t / NFP, t % NFP);
         }
     }
     if constexpr (Cfg::WAVEF) wave_lds_sync();
     else __syncthreads();
-    // one reduction step of the chain, degree dd+1 -> dd, for every (element, face, node)
-    auto chain_step = [&](const T *src, T *dst, int src_off, int dst_off, int dd) {
-        const T inv = T(1) / T(dd + 1);
-        const int nf = nfp2(dd);
-        auto item = [&](int e2, int phi, int beta) {
-            const int ab = dec[np3(dd - 1) + beta];
-            const int a = ab & 255, b = ab >> 8;
-            const int idx = lin2(dd + 1, a, b);
-            const int o1 = idx + 1, o2 = idx + (dd + 2 - b);
-            const T wa = T(a + 1), wb = T(b + 1), wc = T(dd + 1 - a - b);
-            // reads of all F fields first, then the stores (src and dst may be the same array)
-            T x1[F], x2[F], x0[F];
+    // one unnormalised reduction step, degree dd+1 -> dd, at node (a,b) of chain c:
+    //   (a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (dd+1-a-b) x_{ab}   (reduction_multiply! without /(dd+1))
+    // src / dst point at the chain's slab (record 0 of it); idx = lin2(dd+1, a, b)
+    auto chain_item = [&](const T *src, T *dst, int idx, int o2, T wa, T wb, T wc) {
+        T x1[F], x2[F], x0[F], v[F];
+        recv_load<F>(src + (idx + 1) * F, x1);
+        recv_load<F>(src + o2 * F, x2);
+        recv_load<F>(src + idx * F, x0);
 #pragma unroll
-            for (int f = 0; f < F; ++f) {
-                const T *x = src + ((e2 * F + f) * 4 + phi) * QS + src_off;
-                x1[f] = x[o1];
-                x2[f] = x[o2];
-                x0[f] = x[idx];
-            }
-#pragma unroll
-            for (int f = 0; f < F; ++f)
-                dst[((e2 * F + f) * 4 + phi) * QS + dst_off + beta] = fma(wa, x1[f], fma(wb, x2[f], wc * x0[f])) * inv;
-        };
-        if constexpr (Cfg::WAVEF) {
-            for (int q = 0; q < CW; ++q) {
-                const int c = wave_u + NW * q;
-                if (c >= ne * 4) break;
-                for (int beta = tid & 63; beta < nf; beta += 64) item(c >> 2, c & 3, beta);
-            }
-        } else {
-            for (int t = tid; t < ne * 4 * nf; t += NT) item(t / (4 * nf), (t / nf) & 3, t % nf);
-        }
+        for (int f = 0; f < F; ++f) v[f] = fma(wa, x1[f], fma(wb, x2[f], wc * x0[f]));
+        recv_store<F>(dst, v);
     };
 
-    if constexpr (Cfg::GATHER) {
+    if constexpr (Cfg::WAVEF) {
+        // lane -> node (a,b) of the degree N-1 triangle for the whole chain: at step dd the
+        // lane is active while a + b <= dd, and its source index lin2(dd+1,a,b), destination
+        // lin2(dd,a,b) and weight dd+1-a-b each drop by a constant from one step to the next
+        const int lane = tid & 63;
+        const bool lact = lane < nfp2(N - 1);
+        int a = 0, b = 0;
+        if (lact) {
+            const int ab = dec[np3(N - 2) + lane];
+            a = ab & 255;
+            b = ab >> 8;
+        }
+        int idx = lin2(N, a, b), dst = lin2(N - 1, a, b);
+        T wc = T(N - a - b);
+        const T wa = T(a + 1), wb = T(b + 1);
 #pragma unroll
         for (int s = 1; s <= N; ++s) {
-            chain_step(s1, s1, tet(N, s - 1), tet(N, s), N - s);
-            if constexpr (Cfg::WAVEF) wave_lds_sync();
-            else __syncthreads();
+            const int dd = N - s;
+            if (lact && a + b <= dd) {
+                for (int q = 0; q < CW; ++q) {
+                    const int c = wave_u + NW * q;
+                    if (c >= ne * 4) break;
+                    chain_item(s1 + (c * NP + tet(N, s - 1)) * F, s1 + (c * NP + tet(N, s) + dst) * F, idx,
+                               idx + dd + 2 - b, wa, wb, wc);
+                }
+            }
+            wave_lds_sync();
+            idx -= b;
+            dst -= b;
+            wc -= T(1);
         }
-        if constexpr (Cfg::WAVEF) __syncthreads();  // every chain complete before the gather
+        __syncthreads();  // every chain complete before the gather
+    } else {
+        auto chain_step = [&](const T *src, T *dst, int src_off, int dst_off, int dd) {
+            const int nf = nfp2(dd);
+            for (int t = tid; t < ne * 4 * nf; t += NT) {
+                const int c = t / nf, beta = t % nf;
+                const int ab = dec[np3(dd - 1) + beta];
+                const int a = ab & 255, b = ab >> 8;
+                const int idx = lin2(dd + 1, a, b);
+                chain_item(src + (c * QS + src_off) * F, dst + (c * QS + dst_off + beta) * F, idx,
+                           idx + (dd + 2 - b), T(a + 1), T(b + 1), T(dd + 1 - a - b));
+            }
+        };
+        if constexpr (Cfg::GATHER) {
+#pragma unroll
+            for (int s = 1; s <= N; ++s) {
+                chain_step(s1, s1, tet(N, s - 1), tet(N, s), N - s);
+                __syncthreads();
+            }
+        }
+    }
+    if constexpr (Cfg::GATHER) {
         // ---- each node gathers l_slab * E from its four faces
         if (own) {
 #pragma unroll
             for (int o = 0; o < 4; ++o) {
                 const int ff = order[o];
                 const double w = lw[st[ff]];
-                const int off = tet(N, st[ff]) + ix[ff];
+                T v[F];
+                recv_load<F>(s1 + ((el * 4 + ff) * NP + tet(N, st[ff]) + ix[ff]) * F, v);
 #pragma unroll
-                for (int f = 0; f < F; ++f) acc[f] += w * (double)s1[((el * F + f) * 4 + ff) * NP + off];
+                for (int f = 0; f < F; ++f) acc[f] += w * (double)v[f];
             }
         }
     } else {
-        auto accumulate = [&](int s, const T *buf, double w) {
+        auto accumulate = [&](int s, const T *buf) {
             if (!own) return;
+            const double w = lw[s];
 #pragma unroll
             for (int o = 0; o < 4; ++o) {
                 const int ff = order[o];
                 if (st[ff] == s) {
+                    T v[F];
+                    recv_load<F>(buf + ((el * 4 + ff) * NFP + ix[ff]) * F, v);
 #pragma unroll
-                    for (int f = 0; f < F; ++f) acc[f] += w * (double)buf[((el * F + f) * 4 + ff) * NFP + ix[ff]];
+                    for (int f = 0; f < F; ++f) acc[f] += w * (double)v[f];
                 }
             }
         };
-        accumulate(0, s1, 1.0);
+        accumulate(0, s1);
         T *cur = s1, *nxt = s2;
 #pragma unroll
         for (int s = 1; s <= N; ++s) {
-            chain_step(cur, nxt, 0, 0, N - s);
+            const int dd = N - s;
+            const int nf = nfp2(dd);
+            for (int t = tid; t < ne * 4 * nf; t += NT) {
+                const int c = t / nf, beta = t % nf;
+                const int ab = dec[np3(dd - 1) + beta];
+                const int a = ab & 255, b = ab >> 8;
+                const int idx = lin2(dd + 1, a, b);
+                chain_item(cur + c * NFP * F, nxt + (c * NFP + beta) * F, idx, idx + (dd + 2 - b), T(a + 1),
+                           T(b + 1), T(dd + 1 - a - b));
+            }
             __syncthreads();
-            accumulate(s, nxt, lj(N, s));
+            accumulate(s, nxt);
             T *tmp = cur;
             cur = nxt;
             nxt = tmp;

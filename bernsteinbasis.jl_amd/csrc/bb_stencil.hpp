@@ -148,6 +148,103 @@ template <typename T> __device__ __forceinline__ T l0_eval(const T *x, const L0P
     return T(0.5) * s;
 }
 
+// The F fields of one node stored next to each other (LDS records of F values): one
+// address per node, read and written as 16-byte words where the record size is a multiple
+// of 16 bytes, else as 8-byte words (p aligned to that word size)
+template <int F, typename T> __device__ __forceinline__ void recv_load(const T *p, T (&v)[F]) {
+    constexpr int W = (F * (int)sizeof(T)) % 16 == 0 ? 16 / (int)sizeof(T) : 8 / (int)sizeof(T);
+    if constexpr (W == 1 || F % W != 0) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) v[f] = p[f];
+    } else {
+        typedef T V __attribute__((ext_vector_type(W)));
+        const V *q = reinterpret_cast<const V *>(p);
+#pragma unroll
+        for (int c = 0; c < F / W; ++c) {
+            const V x = q[c];
+#pragma unroll
+            for (int w = 0; w < W; ++w) v[c * W + w] = x[w];
+        }
+    }
+}
+template <int F, typename T> __device__ __forceinline__ void recv_store(T *p, const T (&v)[F]) {
+    constexpr int W = (F * (int)sizeof(T)) % 16 == 0 ? 16 / (int)sizeof(T) : 8 / (int)sizeof(T);
+    if constexpr (W == 1 || F % W != 0) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) p[f] = v[f];
+    } else {
+        typedef T V __attribute__((ext_vector_type(W)));
+        V *q = reinterpret_cast<V *>(p);
+#pragma unroll
+        for (int c = 0; c < F / W; ++c) {
+            V x;
+#pragma unroll
+            for (int w = 0; w < W; ++w) x[w] = v[c * W + w];
+            q[c] = x;
+        }
+    }
+}
+
+// Twice the three reference-coordinate derivatives of G fields of node records of F
+// values (x + F m holds node m; x points at the group's first field): the 13 stencil
+// points of grad_ref_ld, each read once for the group, the four weighted differences per
+// derivative in the same order, without the factor 1/2 (the caller folds it into the
+// geometric factors)
+template <int N, int F, int G, typename T>
+__device__ __forceinline__ void grad2_rec(const T *x, int idx, int i, int j, int k, T (&dr)[G], T (&ds)[G],
+                                          T (&dt)[G]) {
+    const int l = N - i - j - k;
+    const int rl = N - k - j + 1;
+    const int sk = (N - k + 1) * (N - k + 2) / 2;
+    const int skm = (N - k + 2) * (N - k + 3) / 2;
+    const int km0 = idx - skm + j, kp0 = idx + sk - j;
+    const T ti = T(i), tj = T(j), tk = T(k), tl = T(l);
+    T xc[G], xim[G], xjm[G], xkm[G], y[G], z[G];
+    recv_load<G>(x + F * idx, xc);
+    recv_load<G>(x + F * (i > 0 ? idx - 1 : idx), xim);
+    recv_load<G>(x + F * (j > 0 ? idx - rl - 1 : idx), xjm);
+    recv_load<G>(x + F * (k > 0 ? km0 : idx), xkm);
+    // d/dr: i (x_c - x_{i-1}) + j (x_{i+1,j-1} - x_{j-1}) + k (x_{i+1,k-1} - x_{k-1}) + l (x_{i+1} - x_c)
+    recv_load<G>(x + F * (j > 0 ? idx - rl : idx), y);
+    recv_load<G>(x + F * (k > 0 ? km0 + 1 : idx), z);
+#pragma unroll
+    for (int f = 0; f < G; ++f) {
+        T v = ti * (xc[f] - xim[f]);
+        v += tj * (y[f] - xjm[f]);
+        v += tk * (z[f] - xkm[f]);
+        dr[f] = v;
+    }
+    recv_load<G>(x + F * (l > 0 ? idx + 1 : idx), y);
+#pragma unroll
+    for (int f = 0; f < G; ++f) dr[f] += tl * (y[f] - xc[f]);
+    // d/ds: i (x_{i-1,j+1} - x_{i-1}) + j (x_c - x_{j-1}) + k (x_{j+1,k-1} - x_{k-1}) + l (x_{j+1} - x_c)
+    recv_load<G>(x + F * (i > 0 ? idx + rl - 1 : idx), y);
+    recv_load<G>(x + F * (k > 0 ? km0 + rl + 1 : idx), z);
+#pragma unroll
+    for (int f = 0; f < G; ++f) {
+        T v = ti * (y[f] - xim[f]);
+        v += tj * (xc[f] - xjm[f]);
+        v += tk * (z[f] - xkm[f]);
+        ds[f] = v;
+    }
+    recv_load<G>(x + F * (l > 0 ? idx + rl : idx), y);
+#pragma unroll
+    for (int f = 0; f < G; ++f) ds[f] += tl * (y[f] - xc[f]);
+    // d/dt: i (x_{i-1,k+1} - x_{i-1}) + j (x_{j-1,k+1} - x_{j-1}) + k (x_c - x_{k-1}) + l (x_{k+1} - x_c)
+    recv_load<G>(x + F * (i > 0 ? kp0 - 1 : idx), y);
+    recv_load<G>(x + F * (j > 0 ? kp0 - rl : idx), z);
+#pragma unroll
+    for (int f = 0; f < G; ++f) {
+        T v = ti * (y[f] - xim[f]);
+        v += tj * (z[f] - xjm[f]);
+        v += tk * (xc[f] - xkm[f]);
+        dt[f] = v;
+    }
+    recv_load<G>(x + F * (l > 0 ? kp0 : idx), y);
+#pragma unroll
+    for (int f = 0; f < G; ++f) dt[f] += tl * (y[f] - xc[f]);
+}
+
 // (E_{d}^{d+1})^T x: degree d+1 -> degree d at (a,b), reduction_multiply!
 // (src/BernsteinLift.jl:66-90): ((a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (d+1-a-b) x_{ab}) / (d+1)
 template <typename T> __device__ __forceinline__ T reduce_apply(const T *x, int d, int a, int b, T inv) {
