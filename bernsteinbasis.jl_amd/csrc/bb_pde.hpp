@@ -34,9 +34,11 @@ template <> struct Pde<BB_PDE_WAVE> {
     // interface flux, examples/3d_wave.jl:45-54 (kind 0 verbatim; kind 1: every term
     // carries Jf^1, i.e. d_p -> Jf d_p and (n.d) -> (n.d)/Jf in the penalty).  Jf = |nJ| and
     // invJf = 1/Jf come from the caller (per face of an affine element).
+    // (sums of products are written as explicit fma, so that every kernel that inlines this
+    // rounds the same way whatever the compiler's contraction choices)
     template <typename T>
     __device__ static void flux(int kind, const T *d, T nx, T ny, T nz, T Jf, T invJf, T *o) {
-        T ndotdU = nx * d[1] + ny * d[2] + nz * d[3];
+        const T ndotdU = fma(nz, d[3], fma(ny, d[2], nx * d[1]));
         if (kind == BB_FLUX_REFERENCE) {
             o[0] = T(0.5) * (d[0] - ndotdU);
             o[1] = T(0.5) * ((ndotdU - d[0]) * nx);
@@ -44,7 +46,7 @@ template <> struct Pde<BB_PDE_WAVE> {
             o[3] = T(0.5) * ((ndotdU - d[0]) * nz);
         } else {
             T un = ndotdU * invJf;
-            o[0] = T(0.5) * (Jf * d[0] - ndotdU);
+            o[0] = T(0.5) * fma(Jf, d[0], -ndotdU);
             o[1] = T(0.5) * ((un - d[0]) * nx);
             o[2] = T(0.5) * ((un - d[0]) * ny);
             o[3] = T(0.5) * ((un - d[0]) * nz);
@@ -72,24 +74,24 @@ template <> struct Pde<BB_PDE_MAXWELL> {
         }
     }
     // examples/3d_maxwell.jl:189-207 (kind 1: penalties d -> Jf d, (n.d)n -> (n.d)n/Jf)
+    // 1/2 (fx(d) nx + fy(d) ny + fz(d) nz + d - pen): the two non-zero entries of
+    // fx nx + fy ny + fz nz per field (n x E and -n x H), as explicit fma; the factor 1/2 is
+    // applied last (exact)
     template <typename T>
     __device__ static void flux(int kind, const T *d, T nx, T ny, T nz, T Jf, T invJf, T *o) {
-        T ndH = nx * d[0] + ny * d[1] + nz * d[2];
-        T ndE = nx * d[3] + ny * d[4] + nz * d[5];
+        T ndH = fma(nz, d[2], fma(ny, d[1], nx * d[0]));
+        T ndE = fma(nz, d[5], fma(ny, d[4], nx * d[3]));
         if (kind != BB_FLUX_REFERENCE) {
             ndH = ndH * invJf;
             ndE = ndE * invJf;
         }
-        T fx[6], fy[6], fz[6];
-        fdir(0, d, fx);
-        fdir(1, d, fy);
-        fdir(2, d, fz);
-        T pen[6] = {ndH * nx, ndH * ny, ndH * nz, ndE * nx, ndE * ny, ndE * nz};
-        T h = T(0.5);
+        const T cr[6] = {fma(d[4], nz, -d[5] * ny), fma(-d[3], nz, d[5] * nx), fma(d[3], ny, -d[4] * nx),
+                         fma(-d[1], nz, d[2] * ny), fma(d[0], nz, -d[2] * nx), fma(-d[0], ny, d[1] * nx)};
+        const T pen[6] = {ndH * nx, ndH * ny, ndH * nz, ndE * nx, ndE * ny, ndE * nz};
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
-            T jd = kind != BB_FLUX_REFERENCE ? Jf * d[f] : d[f];
-            o[f] = h * fx[f] * nx + h * fy[f] * ny + h * fz[f] * nz + h * jd - h * pen[f];
+            const T jd = kind != BB_FLUX_REFERENCE ? Jf * d[f] : d[f];
+            o[f] = T(0.5) * ((cr[f] + jd) - pen[f]);
         }
     }
 };

@@ -17,7 +17,7 @@ struct bb_solver {
     int64_t K = 0;
     double sign[6] = {1, 1, 1, 1, 1, 1};
     // mesh
-    double *geo = nullptr, *J = nullptr, *nJ = nullptr;  // nJ: [K][4][5] nxJ nyJ nzJ Jf 1/Jf
+    double *geo = nullptr, *J = nullptr, *invJ = nullptr, *nJ = nullptr;  // nJ: [K][4][5] nxJ nyJ nzJ Jf 1/Jf
     int2 *conn = nullptr;
     uint16_t *tab = nullptr;
     float *l0a = nullptr;  // L0 as MFMA A operands (mixed precision, matrix-core L0 tilings)
@@ -97,7 +97,7 @@ static int launch_stage(bb_solver *s, int prec, int tiling, const StageArgs &a) 
 static StageArgs stage_args(bb_solver *s, int prec) {
     StageArgs a{};
     a.geo = s->geo;
-    a.J = s->J;
+    a.invJ = s->invJ;
     a.nJ = s->nJ;
     a.conn = s->conn;
     a.tab = s->tab;
@@ -295,7 +295,10 @@ int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, cons
         delete s;
         return bb_fail(ctx, BB_ERR_STATE, "L0 has entries outside the band the matrix-core kernel assumes (N=%d)", N);
     }
+    std::vector<double> invJ(K);
+    for (int64_t e2 = 0; e2 < K; ++e2) invJ[e2] = 1.0 / J_host[e2];
     if ((e = hipMalloc(&s->geo, K * 9 * sizeof(double))) || (e = hipMalloc(&s->J, K * sizeof(double))) ||
+        (e = hipMalloc(&s->invJ, K * sizeof(double))) ||
         (e = hipMalloc(&s->nJ, K * 20 * sizeof(double))) || (e = hipMalloc(&s->conn, K * 4 * sizeof(int2))) ||
         (e = hipMalloc(&s->tab, tab.size() * sizeof(uint16_t))) ||
         (e = hipMalloc(&s->l0a, l0a.size() * sizeof(float))) ||
@@ -303,6 +306,7 @@ int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, cons
         return fail(e);
     if ((e = hipMemcpy(s->geo, geo_host, K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->J, J_host, K * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->invJ, invJ.data(), K * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->nJ, nj5.data(), K * 20 * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->conn, conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->tab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice)) ||
@@ -320,7 +324,7 @@ int bb_solver_destroy(bb_solver *s) {
     if (!s) return BB_ERR_NULL;
     hipStreamSynchronize(s->ctx->stream);
     drop_graphs(s);
-    void *ptrs[] = {s->geo, s->J, s->nJ, s->conn, s->tab, s->l0a, s->mapP, s->bcn, s->nx, s->ny, s->nz, s->uM, s->flux, s->du,
+    void *ptrs[] = {s->geo, s->J, s->invJ, s->nJ, s->conn, s->tab, s->l0a, s->mapP, s->bcn, s->nx, s->ny, s->nz, s->uM, s->flux, s->du,
                     s->q[0], s->q[1], s->res64, s->res32};
     for (void *p : ptrs)
         if (p) hipFree(p);

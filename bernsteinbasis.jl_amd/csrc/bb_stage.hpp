@@ -9,7 +9,7 @@
 // Data in HBM (per element, all contiguous, element-major):
 //   q_in, q_out [K][F][Np] FP64 (ping-pong: neighbours read the old stage while the new
 //   one is written), res [K][F][Np] (FP64, or FP32 in mixed precision), geo [K][9],
-//   J [K], nJ [K][4][3], conn [K][4] = {neighbour, face | perm << 2 | bc << 5}.
+//   1/J [K], nJ [K][4][5], conn [K][4] = {neighbour, face | perm << 2 | bc << 5}.
 //
 // LDS (T = FP64, or FP32 in mixed precision), every array as node records of F values:
 //   s1: the element's coefficients, later chain buffer A
@@ -31,7 +31,7 @@ struct StageArgs {
     double *q_out;
     void *res;
     const double *geo;
-    const double *J;
+    const double *invJ;    // [K] 1/J
     const double *nJ;      // [K][4][5]: nxJ nyJ nzJ Jf 1/Jf per face
     const int2 *conn;
     const uint16_t *tab;   // index tables of degree N (stage_tables)
@@ -87,7 +87,11 @@ constexpr int kTraceMax = 66;  // face nodes of the highest degree of a fused ke
 //             its last reduction step, so that these phases need only wave-level ordering
 //             (no workgroup barrier between them); the element state gets its own LDS
 //             buffer, since fast waves overwrite chain slabs while slow ones still read it.
-enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1, LIFT_GATHER_MFMA = 2, LIFT_WAVE = 3 };
+//  WAVE_GT  : WAVE with the index tables read from global memory (L2-resident; every
+//             entry is read once per workgroup, and the chain steps by increments), which
+//             leaves the LDS to the state and chains and admits more workgroups per CU
+//             where the LDS limits them.
+enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1, LIFT_GATHER_MFMA = 2, LIFT_WAVE = 3, LIFT_WAVE_GT = 4 };
 
 
 template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = false> struct StageCfg {
@@ -97,16 +101,19 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int NT = ((EB * NP + 63) / 64) * 64;
     // PINGPONG: s1 = state, then chain A; s2 = flux, then chain B
     // GATHER:   s1 = state, then all chain vectors (EB F 4 Np); s2 = flux
-    static constexpr bool GATHER = LM == LIFT_GATHER || LM == LIFT_GATHER_MFMA || LM == LIFT_WAVE;
-    static constexpr bool WAVEF = LM == LIFT_WAVE;
+    static constexpr bool WAVEF = LM == LIFT_WAVE || LM == LIFT_WAVE_GT;
+    static constexpr bool GATHER = LM == LIFT_GATHER || LM == LIFT_GATHER_MFMA || WAVEF;
     static constexpr int R0 = WAVEF ? EB * F * NP : 0;  // separate state buffer (WAVE)
     static constexpr int R1 = GATHER ? EB * F * 4 * NP
                               : ((EB * F * NP > EB * F * 4 * NFP) ? EB * F * NP : EB * F * 4 * NFP);
     static constexpr int R2 = EB * F * 4 * NFP;
-    static constexpr int TAB = 2 * NP + 28 * NFP;  // ijk, dec, own face nodes, neighbour face nodes
+    static constexpr int TABG = (2 * NP + 28 * NFP + 3) / 4 * 4;  // ijk, dec, own / neighbour face nodes
+    static constexpr int TAB = TABG + 4 * NP;                      // then the gather offsets
     static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
     static constexpr int R3 = MD ? EB * 4 * F * kTraceMax : 0;  // neighbour traces of other degrees
-    static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (size_t)TAB16 * 16 + (N + 1) * 8;
+    // index tables in LDS, or read from global memory (WAVE_GT)
+    static constexpr bool TABL = LM != LIFT_WAVE_GT;
+    static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (TABL ? (size_t)TAB16 * 16 : 0) + (N + 1) * 8;
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
     static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024 && (LM != LIFT_GATHER_MFMA || MIXED);
@@ -165,9 +172,13 @@ __host__ __device__ constexpr double lw_chain(int N, int s) {
 //   [Np, 2 Np)         entry t of a degree-d triangle (d = 0..N, at offset Np(d-1)) -> a | b << 8
 //   [2 Np, +4 Nfp)     own face f, node m -> volume index
 //   [.., +24 Nfp)      neighbour face g, permutation p, my node m -> neighbour's volume index
+//   [G, G + 4 Np)      (G = the above rounded up to 4 entries) node n, face f -> where face f's
+//                      chain lands on n: tet(N, slab) + index in the slab (f1 slab j at (i,k),
+//                      f2 slab l at (j,k), f3 slab i at (j,k), f4 slab k at (i,j))
 inline std::vector<uint16_t> stage_tables(int N) {
     const int NP = np3(N), NFP = nfp2(N);
-    std::vector<uint16_t> t((2 * NP + 28 * NFP + 7) / 8 * 8, 0);  // padded to 16-byte words
+    const int G = (2 * NP + 28 * NFP + 3) / 4 * 4;
+    std::vector<uint16_t> t((G + 4 * NP + 7) / 8 * 8, 0);  // padded to 16-byte words
     int r = 0;
     for (int k = 0; k <= N; ++k)
         for (int j = 0; j <= N - k; ++j)
@@ -185,6 +196,16 @@ inline std::vector<uint16_t> stage_tables(int N) {
                     const int abc[3] = {a, b, N - a - b};
                     t[r++] = (uint16_t)face_vidx(N, g, abc[perm_entry(p, 0)], abc[perm_entry(p, 1)]);
                 }
+    r = G;
+    for (int k = 0; k <= N; ++k)
+        for (int j = 0; j <= N - k; ++j)
+            for (int i = 0; i <= N - j - k; ++i) {
+                const int l = N - i - j - k;
+                t[r++] = (uint16_t)(tet(N, j) + lin2(N - j, i, k));
+                t[r++] = (uint16_t)(tet(N, l) + lin2(N - l, j, k));
+                t[r++] = (uint16_t)(tet(N, i) + lin2(N - i, j, k));
+                t[r++] = (uint16_t)(tet(N, k) + lin2(N - k, i, j));
+            }
     return t;
 }
 
@@ -207,8 +228,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     __shared__ __align__(16) T s0[Cfg::WAVEF ? Cfg::R0 : 1];
     T *sst = Cfg::WAVEF ? s0 : s1;  // the element state in LDS: [element][node][field]
     __shared__ __align__(16) T s3[MD ? Cfg::R3 : 1];
-    __shared__ uint4 tab16[Cfg::TAB16];
-    const uint16_t *tab = reinterpret_cast<const uint16_t *>(tab16);
+    __shared__ uint4 tab16[Cfg::TABL ? Cfg::TAB16 : 1];
+    const uint16_t *tab = Cfg::TABL ? reinterpret_cast<const uint16_t *>(tab16) : A.tab;
     __shared__ double lw[N + 1];  // slab weights of the unnormalised chain (lw_chain)
     const uint16_t *ijk = tab, *dec = tab + NP, *fvid = tab + 2 * NP, *nvid = tab + 2 * NP + 4 * NFP;
 
@@ -228,7 +249,14 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     // LDS work, so that its latency overlaps the volume phase:
     //   index tables -> LDS (one 16-byte word per thread), the element state and the RK
     //   residual of the owned node, and the neighbour traces of this thread's face nodes
-    for (int t = tid; t < Cfg::TAB16; t += NT) tab16[t] = reinterpret_cast<const uint4 *>(A.tab)[t];
+    int pk = 0;   // this node's (i, j, k)
+    uint2 gp{};   // where the four faces' chains land on this node
+    if constexpr (Cfg::TABL) {
+        for (int t = tid; t < Cfg::TAB16; t += NT) tab16[t] = reinterpret_cast<const uint4 *>(A.tab)[t];
+    } else {
+        pk = tab[n];
+        if constexpr (Cfg::GATHER) gp = reinterpret_cast<const uint2 *>(tab + Cfg::TABG)[n];
+    }
     if (tid <= N) {
 #pragma unroll
         for (int s = 0; s <= N; ++s)
@@ -278,6 +306,23 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             return t < ne * 4 * NFP;
         }
     };
+    // WAVE_GT (tables in global memory): the per-lane table entries of the later phases are
+    // read here with the other global reads -- own face node of each flux item, the face
+    // node (a,b) of each L0 item and the chain lane's node of the degree N-1 triangle
+    int fvp[RF], l0p[MI], chp = 0;
+    if constexpr (!Cfg::TABL) {
+#pragma unroll
+        for (int r = 0; r < RF; ++r) {
+            int e2, phi, m;
+            fvp[r] = fitem(r, e2, phi, m) ? fvid[phi * NFP + m] : 0;
+        }
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi) {
+            const int beta = mi * 64 + (tid & 63);
+            l0p[mi] = beta < NFP ? dec[np3(N - 1) + beta] : 0;
+        }
+        if ((tid & 63) < nfp2(N - 1)) chp = dec[np3(N - 2) + (tid & 63)];
+    }
     T nb[RF][F];
     int2 cnv[RF];
 #pragma unroll
@@ -308,7 +353,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 T v[F];
 #pragma unroll
                 for (int f = 0; f < F; ++f) v[f] = T(qn[f * npn]);
-                recv_store<F>(s3 + ((e2 * 4 + phi) * kTraceMax + jn) * F, v);
+                recv_store<F>(s3 + __mul24((e2 * 4 + phi) * kTraceMax + jn, F), v);
             }
         }
     }
@@ -316,7 +361,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         T v[F];
 #pragma unroll
         for (int f = 0; f < F; ++f) v[f] = T(qv[f]);
-        recv_store<F>(sst + (el * NP + n) * F, v);
+        recv_store<F>(sst + __mul24(el * NP + n, F), v);
     }
     __syncthreads();  // state visible
 
@@ -324,7 +369,10 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     T vol[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) vol[f] = T(0);
-    const int pk = ijk[n];
+    if constexpr (Cfg::TABL) {
+        pk = ijk[n];
+        if constexpr (Cfg::GATHER) gp = reinterpret_cast<const uint2 *>(tab + Cfg::TABG)[n];
+    }
     const int i = pk & 15, j = (pk >> 4) & 15, k = pk >> 8;
     if (own) {
         // halved geometric factors: grad2_rec returns twice the reference derivatives
@@ -334,16 +382,16 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         for (int c = 0; c < 9; ++c) gh[c] = T(0.5 * g[c]);
         // fields in groups of G (one 16- or 8-byte word per stencil point and group)
         constexpr int G = (sizeof(T) == 4 && F % 4 == 0) ? 4 : 2;
-        const T *xs = sst + el * NP * F;
+        const T *xs = sst + __mul24(el, NP * F);
 #pragma unroll
         for (int g0 = 0; g0 < F; g0 += G) {
             T dr[G], ds[G], dt[G];
             grad2_rec<N, F, G, T>(xs + g0, n, i, j, k, dr, ds, dt);
 #pragma unroll
             for (int q = 0; q < G; ++q) {
-                const T gx = gh[0] * dr[q] + gh[1] * ds[q] + gh[2] * dt[q];
-                const T gy = gh[3] * dr[q] + gh[4] * ds[q] + gh[5] * dt[q];
-                const T gz = gh[6] * dr[q] + gh[7] * ds[q] + gh[8] * dt[q];
+                const T gx = fma(gh[2], dt[q], fma(gh[1], ds[q], gh[0] * dr[q]));
+                const T gy = fma(gh[5], dt[q], fma(gh[4], ds[q], gh[3] * dr[q]));
+                const T gz = fma(gh[8], dt[q], fma(gh[7], ds[q], gh[6] * dr[q]));
                 P::template vol_add<T>(g0 + q, gx, gy, gz, vol);
             }
             // one group at a time: hoisting the next group's reads above this group's
@@ -357,9 +405,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     for (int r = 0; r < RF; ++r) {
         int e2, phi, m;
         if (fitem(r, e2, phi, m)) {
-            const int vi = fvid[phi * NFP + m];
+            const int vi = Cfg::TABL ? fvid[phi * NFP + m] : fvp[r];
             T um[F], d[F], o[F];
-            recv_load<F>(sst + (e2 * NP + vi) * F, um);
+            recv_load<F>(sst + __mul24(e2 * NP + vi, F), um);
             if (MD && ((cnv[r].y >> 5) & 7) == 0 && ((cnv[r].y >> 8) & 15) != N) {
                 // the neighbour's trace of degree Nn brought to degree N (in the neighbour's
                 // face orientation), read at this node through the face permutation
@@ -369,7 +417,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int q0 = perm_entry(p, 0), q1 = perm_entry(p, 1);
                 const int a2 = q0 == 0 ? a : (q0 == 1 ? b : c), b2 = q1 == 0 ? a : (q1 == 1 ? b : c);
                 const int mp = lin2(N, a2, b2);
-                const T *tn = s3 + (e2 * 4 + phi) * kTraceMax * F;
+                const T *tn = s3 + __mul24(e2 * 4 + phi, kTraceMax * F);
                 T up[F], tv[F];
 #pragma unroll
                 for (int f = 0; f < F; ++f) up[f] = T(0);
@@ -379,18 +427,18 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                     const int j1 = rp[1];
                     for (int jj = rp[0]; jj < j1; ++jj) {
                         const T w = MIXED ? T(A.evf[jj]) : T(A.evd[jj]);
-                        recv_load<F>(tn + A.ecol[jj] * F, tv);
+                        recv_load<F>(tn + __mul24(A.ecol[jj], F), tv);
 #pragma unroll
-                        for (int f = 0; f < F; ++f) up[f] += w * tv[f];
+                        for (int f = 0; f < F; ++f) up[f] = fma(w, tv[f], up[f]);
                     }
                 } else {
                     // L2 projection from a higher degree: a dense row
                     const int64_t ro = A.troff[11 * N + Nn] + (int64_t)mp * nfn;
                     for (int jn = 0; jn < nfn; ++jn) {
                         const T w = MIXED ? T(A.trf[ro + jn]) : T(A.tr[ro + jn]);
-                        recv_load<F>(tn + jn * F, tv);
+                        recv_load<F>(tn + __mul24(jn, F), tv);
 #pragma unroll
-                        for (int f = 0; f < F; ++f) up[f] += w * tv[f];
+                        for (int f = 0; f < F; ++f) up[f] = fma(w, tv[f], up[f]);
                     }
                 }
 #pragma unroll
@@ -404,7 +452,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             }
             const double *nj = A.nJ + ((e0 + e2) * 4 + phi) * 5;
             P::template flux<T>(A.flux_kind, d, T(nj[0]), T(nj[1]), T(nj[2]), T(nj[3]), T(nj[4]), o);
-            recv_store<F>(s2 + ((e2 * 4 + phi) * NFP + m) * F, o);
+            recv_store<F>(s2 + __mul24((e2 * 4 + phi) * NFP + m, F), o);
         }
     }
     if constexpr (Cfg::WAVEF) wave_lds_sync();
@@ -477,54 +525,59 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             }
         }
     } else {
-        auto l0_item = [&](int c, int beta) {
-            const int ab = dec[np3(N - 1) + beta];
-            const L0Pt<T> pt = l0_prep<N, T>(ab & 255, ab >> 8);
-            const T *x = s2 + c * NFP * F;
+        auto l0_item = [&](int c, int beta, int ab) {
+            const L0Rec<T> pt = l0_prep_rec<N, F, T>(ab & 255, ab >> 8);
+            const T *x = s2 + __mul24(c, NFP * F);
             T v[F], y[F];
-            recv_load<F>(x + F * pt.i0, y);
+            // the 7-point sum in the order of l0_eval, as explicit fma
+            recv_load<F>(x + pt.o0, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = pt.w0 * y[f];
-            recv_load<F>(x + F * pt.i1, y);
+            recv_load<F>(x + pt.o1, y);
 #pragma unroll
-            for (int f = 0; f < F; ++f) v[f] += pt.w1 * y[f];
-            recv_load<F>(x + F * pt.i2, y);
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w1, y[f], v[f]);
+            recv_load<F>(x + pt.o2, y);
 #pragma unroll
-            for (int f = 0; f < F; ++f) v[f] += pt.w2 * y[f];
-            recv_load<F>(x + F * pt.i3, y);
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w2, y[f], v[f]);
+            recv_load<F>(x + pt.o3, y);
 #pragma unroll
-            for (int f = 0; f < F; ++f) v[f] += pt.w3 * y[f];
-            recv_load<F>(x + F * pt.i4, y);
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w3, y[f], v[f]);
+            recv_load<F>(x + pt.o4, y);
 #pragma unroll
-            for (int f = 0; f < F; ++f) v[f] += pt.w4 * y[f];
-            recv_load<F>(x + F * pt.i5, y);
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w4, y[f], v[f]);
+            recv_load<F>(x + pt.o5, y);
 #pragma unroll
-            for (int f = 0; f < F; ++f) v[f] += pt.w5 * y[f];
-            recv_load<F>(x + F * pt.i6, y);
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w5, y[f], v[f]);
+            recv_load<F>(x + pt.o6, y);
 #pragma unroll
-            for (int f = 0; f < F; ++f) v[f] += pt.w6 * y[f];
-            recv_store<F>(s1 + (c * QS + beta) * F, v);
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w6, y[f], v[f]);
+            recv_store<F>(s1 + __mul24(c * QS + beta, F), v);
         };
         if constexpr (Cfg::WAVEF) {
             for (int q = 0; q < CW; ++q) {
                 const int c = wave_u + NW * q;
                 if (c >= ne * 4) break;
-                for (int beta = tid & 63; beta < NFP; beta += 64) l0_item(c, beta);
+#pragma unroll
+                for (int mi = 0; mi < MI; ++mi) {
+                    const int beta = mi * 64 + (tid & 63);
+                    if (beta < NFP) l0_item(c, beta, Cfg::TABL ? (int)dec[np3(N - 1) + beta] : l0p[mi]);
+                }
             }
         } else {
-            for (int t = tid; t < ne * 4 * NFP; t += NT) l0_item(t / NFP, t % NFP);
+            for (int t = tid; t < ne * 4 * NFP; t += NT) l0_item(t / NFP, t % NFP, dec[np3(N - 1) + t % NFP]);
         }
     }
     if constexpr (Cfg::WAVEF) wave_lds_sync();
     else __syncthreads();
     // one unnormalised reduction step, degree dd+1 -> dd, at node (a,b) of chain c:
     //   (a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (dd+1-a-b) x_{ab}   (reduction_multiply! without /(dd+1))
-    // src / dst point at the chain's slab (record 0 of it); idx = lin2(dd+1, a, b)
-    auto chain_item = [&](const T *src, T *dst, int idx, int o2, T wa, T wb, T wc) {
+    // src points at the chain's source slab, dst at the output record; record offsets
+    // idxF = F lin2(dd+1, a, b) and o2F = F (lin2(dd+1, a, b) + dd + 2 - b)
+    auto chain_item = [&](const T *src, T *dst, int idxF, int o2F, T wa, T wb, T wc) {
         T x1[F], x2[F], x0[F], v[F];
-        recv_load<F>(src + (idx + 1) * F, x1);
-        recv_load<F>(src + o2 * F, x2);
-        recv_load<F>(src + idx * F, x0);
+        recv_load<F>(src + idxF + F, x1);
+        recv_load<F>(src + o2F, x2);
+        recv_load<F>(src + idxF, x0);
 #pragma unroll
         for (int f = 0; f < F; ++f) v[f] = fma(wa, x1[f], fma(wb, x2[f], wc * x0[f]));
         recv_store<F>(dst, v);
@@ -538,11 +591,13 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         const bool lact = lane < nfp2(N - 1);
         int a = 0, b = 0;
         if (lact) {
-            const int ab = dec[np3(N - 2) + lane];
+            const int ab = Cfg::TABL ? (int)dec[np3(N - 2) + lane] : chp;
             a = ab & 255;
             b = ab >> 8;
         }
-        int idx = lin2(N, a, b), dst = lin2(N - 1, a, b);
+        const int bF = __mul24(b, F);
+        int idxF = __mul24(lin2(N, a, b), F), dstF = __mul24(lin2(N - 1, a, b), F);
+        int o2F = idxF + __mul24(N + 1 - b, F);
         T wc = T(N - a - b);
         const T wa = T(a + 1), wb = T(b + 1);
 #pragma unroll
@@ -552,13 +607,14 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 for (int q = 0; q < CW; ++q) {
                     const int c = wave_u + NW * q;
                     if (c >= ne * 4) break;
-                    chain_item(s1 + (c * NP + tet(N, s - 1)) * F, s1 + (c * NP + tet(N, s) + dst) * F, idx,
-                               idx + dd + 2 - b, wa, wb, wc);
+                    chain_item(s1 + (c * NP + tet(N, s - 1)) * F, s1 + (c * NP + tet(N, s)) * F + dstF, idxF, o2F,
+                               wa, wb, wc);
                 }
             }
             wave_lds_sync();
-            idx -= b;
-            dst -= b;
+            idxF -= bF;
+            dstF -= bF;
+            o2F -= bF + F;
             wc -= T(1);
         }
         __syncthreads();  // every chain complete before the gather
@@ -569,9 +625,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int c = t / nf, beta = t % nf;
                 const int ab = dec[np3(dd - 1) + beta];
                 const int a = ab & 255, b = ab >> 8;
-                const int idx = lin2(dd + 1, a, b);
-                chain_item(src + (c * QS + src_off) * F, dst + (c * QS + dst_off + beta) * F, idx,
-                           idx + (dd + 2 - b), T(a + 1), T(b + 1), T(dd + 1 - a - b));
+                const int idxF = __mul24(lin2(dd + 1, a, b), F);
+                chain_item(src + __mul24(c * QS + src_off, F), dst + __mul24(c * QS + dst_off + beta, F), idxF,
+                           idxF + __mul24(dd + 2 - b, F), T(a + 1), T(b + 1), T(dd + 1 - a - b));
             }
         };
         if constexpr (Cfg::GATHER) {
@@ -585,14 +641,15 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     if constexpr (Cfg::GATHER) {
         // ---- each node gathers l_slab * E from its four faces
         if (own) {
+            const int go[4] = {(int)(gp.x & 0xffff), (int)(gp.x >> 16), (int)(gp.y & 0xffff), (int)(gp.y >> 16)};
 #pragma unroll
             for (int o = 0; o < 4; ++o) {
                 const int ff = order[o];
                 const double w = lw[st[ff]];
                 T v[F];
-                recv_load<F>(s1 + ((el * 4 + ff) * NP + tet(N, st[ff]) + ix[ff]) * F, v);
+                recv_load<F>(s1 + __mul24(el * 4 + ff, NP * F) + __mul24(go[ff], F), v);
 #pragma unroll
-                for (int f = 0; f < F; ++f) acc[f] += w * (double)v[f];
+                for (int f = 0; f < F; ++f) acc[f] = fma(w, (double)v[f], acc[f]);
             }
         }
     } else {
@@ -604,9 +661,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int ff = order[o];
                 if (st[ff] == s) {
                     T v[F];
-                    recv_load<F>(buf + ((el * 4 + ff) * NFP + ix[ff]) * F, v);
+                    recv_load<F>(buf + __mul24((el * 4 + ff) * NFP + ix[ff], F), v);
 #pragma unroll
-                    for (int f = 0; f < F; ++f) acc[f] += w * (double)v[f];
+                    for (int f = 0; f < F; ++f) acc[f] = fma(w, (double)v[f], acc[f]);
                 }
             }
         };
@@ -620,9 +677,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int c = t / nf, beta = t % nf;
                 const int ab = dec[np3(dd - 1) + beta];
                 const int a = ab & 255, b = ab >> 8;
-                const int idx = lin2(dd + 1, a, b);
-                chain_item(cur + c * NFP * F, nxt + (c * NFP + beta) * F, idx, idx + (dd + 2 - b), T(a + 1),
-                           T(b + 1), T(dd + 1 - a - b));
+                const int idxF = __mul24(lin2(dd + 1, a, b), F);
+                chain_item(cur + __mul24(c, NFP * F), nxt + __mul24(c * NFP + beta, F), idxF,
+                           idxF + __mul24(dd + 2 - b, F), T(a + 1), T(b + 1), T(dd + 1 - a - b));
             }
             __syncthreads();
             accumulate(s, nxt);
@@ -634,7 +691,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 
     // ---- epilogue: rhs = (lift + volume) / J, then the RK stage
     if (own) {
-        const double invJ = 1.0 / A.J[eg];
+        const double invJ = A.invJ[eg];
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             const double rhs = (acc[f] + (double)vol[f]) * invJ;
@@ -643,7 +700,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 res[g] = TR(rhs);
             } else {
                 const double dte = MD ? A.dt * A.dte[eg] : A.dt;
-                const double r = A.first ? dte * rhs : A.ca * (double)rv[f] + dte * rhs;
+                const double r = A.first ? dte * rhs : fma(A.ca, (double)rv[f], dte * rhs);
                 const TR rs = TR(r);
                 res[g] = rs;
                 A.q_out[g] = qv[f] + A.cb * (double)rs;
@@ -666,7 +723,7 @@ int stage_launch_wave_md_mixed(int N, int tiling, const StageArgs &a, hipStream_
 constexpr int kStageMaxN = 10;
 // candidate tilings (elements per block, lift organisation), in autotuning order
 constexpr int kStageEB[] = {1, 2, 4};
-constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG, LIFT_GATHER_MFMA, LIFT_WAVE};
+constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG, LIFT_GATHER_MFMA, LIFT_WAVE, LIFT_WAVE_GT};
 
 // tiling code: elements per block + 16 * lift organisation
 constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
@@ -698,8 +755,11 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             using P1 = std::integral_constant<int, LIFT_PINGPONG>;                                     \
             using M2 = std::integral_constant<int, LIFT_GATHER_MFMA>;                                  \
             using W3 = std::integral_constant<int, LIFT_WAVE>;                                         \
+            using W4 = std::integral_constant<int, LIFT_WAVE_GT>;                                      \
             one(std::integral_constant<int, 1>{}, W3{});                                               \
             one(std::integral_constant<int, 2>{}, W3{});                                               \
+            one(std::integral_constant<int, 1>{}, W4{});                                               \
+            one(std::integral_constant<int, 2>{}, W4{});                                               \
             one(std::integral_constant<int, 1>{}, M2{});                                               \
             one(std::integral_constant<int, 2>{}, M2{});                                               \
             one(std::integral_constant<int, 4>{}, M2{});                                               \
@@ -738,6 +798,7 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             using G0 = std::integral_constant<int, LIFT_GATHER>;                                       \
             using P1 = std::integral_constant<int, LIFT_PINGPONG>;                                     \
             using W3 = std::integral_constant<int, LIFT_WAVE>;                                         \
+            using W4 = std::integral_constant<int, LIFT_WAVE_GT>;                                      \
             one(std::integral_constant<int, 1>{}, G0{});                                               \
             one(std::integral_constant<int, 2>{}, G0{});                                               \
             one(std::integral_constant<int, 4>{}, G0{});                                               \
@@ -745,6 +806,7 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             one(std::integral_constant<int, 2>{}, P1{});                                               \
             one(std::integral_constant<int, 4>{}, P1{});                                               \
             one(std::integral_constant<int, 1>{}, W3{});                                               \
+            one(std::integral_constant<int, 1>{}, W4{});                                               \
             return rc;                                                                                 \
         });                                                                                            \
     }                                                                                                  \

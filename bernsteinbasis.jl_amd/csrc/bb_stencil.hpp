@@ -188,8 +188,8 @@ template <int F, typename T> __device__ __forceinline__ void recv_store(T *p, co
 // Twice the three reference-coordinate derivatives of G fields of node records of F
 // values (x + F m holds node m; x points at the group's first field): the 13 stencil
 // points of grad_ref_ld, each read once for the group, the four weighted differences per
-// derivative in the same order, without the factor 1/2 (the caller folds it into the
-// geometric factors)
+// derivative in the same order (as explicit fma, so that every kernel rounds alike),
+// without the factor 1/2 (the caller folds it into the geometric factors)
 template <int N, int F, int G, typename T>
 __device__ __forceinline__ void grad2_rec(const T *x, int idx, int i, int j, int k, T (&dr)[G], T (&ds)[G],
                                           T (&dt)[G]) {
@@ -197,52 +197,66 @@ __device__ __forceinline__ void grad2_rec(const T *x, int idx, int i, int j, int
     const int rl = N - k - j + 1;
     const int sk = (N - k + 1) * (N - k + 2) / 2;
     const int skm = (N - k + 2) * (N - k + 3) / 2;
-    const int km0 = idx - skm + j, kp0 = idx + sk - j;
+    // record offsets (index times F) of the stencil points, 24-bit multiplies only
+    const int cF = __mul24(idx, F), rlF = __mul24(rl, F);
+    const int kmF = cF + __mul24(j - skm, F), kpF = cF + __mul24(sk - j, F);
     const T ti = T(i), tj = T(j), tk = T(k), tl = T(l);
     T xc[G], xim[G], xjm[G], xkm[G], y[G], z[G];
-    recv_load<G>(x + F * idx, xc);
-    recv_load<G>(x + F * (i > 0 ? idx - 1 : idx), xim);
-    recv_load<G>(x + F * (j > 0 ? idx - rl - 1 : idx), xjm);
-    recv_load<G>(x + F * (k > 0 ? km0 : idx), xkm);
+    recv_load<G>(x + cF, xc);
+    recv_load<G>(x + (i > 0 ? cF - F : cF), xim);
+    recv_load<G>(x + (j > 0 ? cF - rlF - F : cF), xjm);
+    recv_load<G>(x + (k > 0 ? kmF : cF), xkm);
     // d/dr: i (x_c - x_{i-1}) + j (x_{i+1,j-1} - x_{j-1}) + k (x_{i+1,k-1} - x_{k-1}) + l (x_{i+1} - x_c)
-    recv_load<G>(x + F * (j > 0 ? idx - rl : idx), y);
-    recv_load<G>(x + F * (k > 0 ? km0 + 1 : idx), z);
+    recv_load<G>(x + (j > 0 ? cF - rlF : cF), y);
+    recv_load<G>(x + (k > 0 ? kmF + F : cF), z);
 #pragma unroll
-    for (int f = 0; f < G; ++f) {
-        T v = ti * (xc[f] - xim[f]);
-        v += tj * (y[f] - xjm[f]);
-        v += tk * (z[f] - xkm[f]);
-        dr[f] = v;
-    }
-    recv_load<G>(x + F * (l > 0 ? idx + 1 : idx), y);
+    for (int f = 0; f < G; ++f) dr[f] = fma(tk, z[f] - xkm[f], fma(tj, y[f] - xjm[f], ti * (xc[f] - xim[f])));
+    recv_load<G>(x + (l > 0 ? cF + F : cF), y);
 #pragma unroll
-    for (int f = 0; f < G; ++f) dr[f] += tl * (y[f] - xc[f]);
+    for (int f = 0; f < G; ++f) dr[f] = fma(tl, y[f] - xc[f], dr[f]);
     // d/ds: i (x_{i-1,j+1} - x_{i-1}) + j (x_c - x_{j-1}) + k (x_{j+1,k-1} - x_{k-1}) + l (x_{j+1} - x_c)
-    recv_load<G>(x + F * (i > 0 ? idx + rl - 1 : idx), y);
-    recv_load<G>(x + F * (k > 0 ? km0 + rl + 1 : idx), z);
+    recv_load<G>(x + (i > 0 ? cF + rlF - F : cF), y);
+    recv_load<G>(x + (k > 0 ? kmF + rlF + F : cF), z);
 #pragma unroll
-    for (int f = 0; f < G; ++f) {
-        T v = ti * (y[f] - xim[f]);
-        v += tj * (xc[f] - xjm[f]);
-        v += tk * (z[f] - xkm[f]);
-        ds[f] = v;
-    }
-    recv_load<G>(x + F * (l > 0 ? idx + rl : idx), y);
+    for (int f = 0; f < G; ++f) ds[f] = fma(tk, z[f] - xkm[f], fma(tj, xc[f] - xjm[f], ti * (y[f] - xim[f])));
+    recv_load<G>(x + (l > 0 ? cF + rlF : cF), y);
 #pragma unroll
-    for (int f = 0; f < G; ++f) ds[f] += tl * (y[f] - xc[f]);
+    for (int f = 0; f < G; ++f) ds[f] = fma(tl, y[f] - xc[f], ds[f]);
     // d/dt: i (x_{i-1,k+1} - x_{i-1}) + j (x_{j-1,k+1} - x_{j-1}) + k (x_c - x_{k-1}) + l (x_{k+1} - x_c)
-    recv_load<G>(x + F * (i > 0 ? kp0 - 1 : idx), y);
-    recv_load<G>(x + F * (j > 0 ? kp0 - rl : idx), z);
+    recv_load<G>(x + (i > 0 ? kpF - F : cF), y);
+    recv_load<G>(x + (j > 0 ? kpF - rlF : cF), z);
 #pragma unroll
-    for (int f = 0; f < G; ++f) {
-        T v = ti * (y[f] - xim[f]);
-        v += tj * (z[f] - xjm[f]);
-        v += tk * (xc[f] - xkm[f]);
-        dt[f] = v;
-    }
-    recv_load<G>(x + F * (l > 0 ? kp0 : idx), y);
+    for (int f = 0; f < G; ++f) dt[f] = fma(tk, xc[f] - xkm[f], fma(tj, z[f] - xjm[f], ti * (y[f] - xim[f])));
+    recv_load<G>(x + (l > 0 ? kpF : cF), y);
 #pragma unroll
-    for (int f = 0; f < G; ++f) dt[f] += tl * (y[f] - xc[f]);
+    for (int f = 0; f < G; ++f) dt[f] = fma(tl, y[f] - xc[f], dt[f]);
+}
+
+// The L0 stencil of l0_prep for node records of F values: record offsets (index times F)
+// of the 7 points and their integer weights (without the factor 1/2 of L0)
+template <typename T> struct L0Rec {
+    int o0, o1, o2, o3, o4, o5, o6;
+    T w0, w1, w2, w3, w4, w5, w6;
+};
+template <int N, int F, typename T> __device__ __forceinline__ L0Rec<T> l0_prep_rec(int a, int b) {
+    const int c = N - a - b;
+    const int cF = __mul24(lin2(N, a, b), F), rlF = __mul24(N - b + 1, F);
+    L0Rec<T> p;
+    p.o0 = cF;
+    p.o1 = b > 0 ? cF - rlF : cF;
+    p.o2 = c > 0 ? cF + F : cF;
+    p.o3 = a > 0 ? cF + rlF - F : cF;
+    p.o4 = c > 0 ? cF + rlF : cF;
+    p.o5 = a > 0 ? cF - F : cF;
+    p.o6 = b > 0 ? cF - rlF - F : cF;
+    p.w0 = T((a + 1) * (a + 1) + (b + 1) * (b + 1) + (c + 1) * (c + 1));
+    p.w1 = T((a + 1) * b);
+    p.w2 = T((a + 1) * c);
+    p.w3 = T((b + 1) * a);
+    p.w4 = T((b + 1) * c);
+    p.w5 = T((c + 1) * a);
+    p.w6 = T((c + 1) * b);
+    return p;
 }
 
 // (E_{d}^{d+1})^T x: degree d+1 -> degree d at (a,b), reduction_multiply!

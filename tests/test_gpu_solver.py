@@ -91,7 +91,7 @@ def test_autotune_and_tiling_invariance(ctx):
     outs = {}
     import bbdg
 
-    for lift in ("gather", "pingpong", "wave"):
+    for lift in ("gather", "pingpong", "wave", "wave_gt"):
         for eb in (1, 2, 4):
             try:
                 s.set_tuning("f64", s.tiling_code(eb, lift))
@@ -102,12 +102,14 @@ def test_autotune_and_tiling_invariance(ctx):
         for eb in (2, 4):
             if (lift, eb) in outs:
                 assert np.array_equal(outs[(lift, eb)], outs[(lift, 1)])
-    # the wave organisation computes every chain item exactly as the gather one
+    # the wave organisations compute every chain item exactly as the gather one
     assert np.array_equal(outs[("wave", 1)], outs[("gather", 1)])
+    assert np.array_equal(outs[("wave_gt", 1)], outs[("gather", 1)])
     t = s.autotune("f64")
-    assert t in [s.tiling_code(e, lm) for e in (1, 2, 4) for lm in ("gather", "pingpong", "wave")]
+    assert t in [s.tiling_code(e, lm) for e in (1, 2, 4) for lm in ("gather", "pingpong", "wave", "wave_gt")]
     tm = s.autotune("mixed")
-    assert tm in [s.tiling_code(e, lm) for e in (1, 2, 4) for lm in ("gather", "pingpong", "gather_mfma", "wave")]
+    assert tm in [s.tiling_code(e, lm) for e in (1, 2, 4)
+                  for lm in ("gather", "pingpong", "gather_mfma", "wave", "wave_gt")]
     assert rel(s.rhs("fused", "f64"), ref) <= TOL64
     for prec in ("f64", "mixed"):
         s.set_tuning(prec, s.tiling_code(1, "pingpong"))
@@ -118,8 +120,8 @@ def test_autotune_and_tiling_invariance(ctx):
 @pytest.mark.parametrize("pde,N,n", [("wave", 2, 3), ("wave", 7, 2), ("wave", 10, 2), ("maxwell", 5, 3),
                                      ("maxwell", 9, 2), ("maxwell", 8, 2)])
 def test_wave_owned_chains(ctx, pde, N, n):
-    """chains owned by single waves (no workgroup barrier between flux, L0 and the chain):
-    FP64 bit-identical to the gather organisation, mixed within its tolerance, for every
+    """chains owned by single waves (no workgroup barrier between flux, L0 and the chain),
+    index tables in LDS or in global memory: FP64 bit-identical to the gather organisation, mixed within its tolerance, for every
     compiled elements-per-workgroup count (partial last workgroups included), RHS and two
     steps"""
     import bbdg
@@ -131,26 +133,27 @@ def test_wave_owned_chains(ctx, pde, N, n):
     ran = 0
     for prec in ("f64", "mixed"):
         for eb in (1, 2):
-            try:
-                s.set_tuning(prec, s.tiling_code(eb, "wave"))
-            except bbdg.BBError:
-                continue  # does not fit in LDS
-            ran += 1
-            got = s.rhs("fused", prec)
-            assert rel(got, ref) <= (TOL64 if prec == "f64" else 2e-5)
-            s.set_state(q0)
-            s.step(dt, 2, "fused", prec)
-            a = s.get_state()
-            s.set_tuning(prec, s.tiling_code(eb, "gather"))
-            s.set_state(q0)
-            s.step(dt, 2, "fused", prec)
-            b = s.get_state()
-            s.set_state(q0)
-            if prec == "f64":
-                assert np.array_equal(a, b)
-                assert np.array_equal(got, s.rhs("fused", prec))
-            else:
-                assert rel(a, b) <= 1e-6
+            for org in ("wave", "wave_gt"):
+                try:
+                    s.set_tuning(prec, s.tiling_code(eb, org))
+                except bbdg.BBError:
+                    continue  # does not fit in LDS
+                ran += 1
+                got = s.rhs("fused", prec)
+                assert rel(got, ref) <= (TOL64 if prec == "f64" else 2e-5)
+                s.set_state(q0)
+                s.step(dt, 2, "fused", prec)
+                a = s.get_state()
+                s.set_tuning(prec, s.tiling_code(eb, "gather"))
+                s.set_state(q0)
+                s.step(dt, 2, "fused", prec)
+                b = s.get_state()
+                s.set_state(q0)
+                if prec == "f64":
+                    assert np.array_equal(a, b)
+                    assert np.array_equal(got, s.rhs("fused", prec))
+                else:
+                    assert rel(a, b) <= 1e-6
     assert ran >= 1
 
 
