@@ -263,6 +263,201 @@ __global__ __launch_bounds__(kLiftW) void k_lift4w(const TI *__restrict__ x, TI 
     }
 }
 
+// Compile-time index tables of degree N for the record kernel below:
+//   ab[np3(d-1) + t] = a | b << 8 for entry t of a degree-d triangle (d = 0..N)
+//   go[4 n + f]      = where face f's chain lands on volume node n: tet(N, slab) + index
+//                      in the slab (f1 slab j at (i,k), f2 slab l at (j,k), f3 slab i at
+//                      (j,k), f4 slab k at (i,j); src/BernsteinLift.jl:202-217)
+//   sl[n]            = the four slabs j | l << 4 | i << 8 | k << 12
+template <int N> struct LiftTab {
+    uint16_t ab[np3(N)];
+    uint16_t go[4 * np3(N)];
+    uint16_t sl[np3(N)];
+    constexpr LiftTab() : ab{}, go{}, sl{} {
+        int r = 0;
+        for (int d = 0; d <= N; ++d)
+            for (int b = 0; b <= d; ++b)
+                for (int a = 0; a <= d - b; ++a) ab[r++] = (uint16_t)(a | (b << 8));
+        int n = 0;
+        for (int k = 0; k <= N; ++k)
+            for (int j = 0; j <= N - k; ++j)
+                for (int i = 0; i <= N - j - k; ++i, ++n) {
+                    const int l = N - i - j - k;
+                    go[4 * n + 0] = (uint16_t)(tet(N, j) + lin2(N - j, i, k));
+                    go[4 * n + 1] = (uint16_t)(tet(N, l) + lin2(N - l, j, k));
+                    go[4 * n + 2] = (uint16_t)(tet(N, i) + lin2(N - i, j, k));
+                    go[4 * n + 3] = (uint16_t)(tet(N, k) + lin2(N - k, i, j));
+                    sl[n] = (uint16_t)(j | (l << 4) | (i << 8) | (k << 12));
+                }
+    }
+};
+template <int N> __device__ constexpr LiftTab<N> kLiftTab = LiftTab<N>();
+
+// Compact field-major layouts with F (1..4, 6) as a template parameter and N <= 10: one
+// block of four waves per element, wave f owning face f's chain for all F fields.  The
+// face data become LDS records of F values (field fastest), so that a stencil point is one
+// address and one or two vector accesses for all fields.  The chain is unnormalised (slab
+// 0 = 2 L0 x, no 1/(d+1) per step; lw_chain carries both factors), each lane keeps its node
+// (a,b) of the degree N-1 triangle for the whole chain and steps its offsets by
+// increments, and the steps of a chain need only wave-level ordering.  Every slab of every
+// chain stays in LDS; each thread then gathers its volume nodes' four face values in the
+// order G4 + G1 + G2 + G3 with the weights in TA.
+constexpr int kLiftR = 256;
+template <int N, int F, typename TI, typename TC> struct LiftRCfg {
+    static constexpr int NP = np3(N), NFP = nfp2(N);
+    static constexpr int RS = rec_stride<TC>(F);  // LDS record stride (values)
+    static constexpr size_t lds = (size_t)4 * (NP + NFP) * RS * sizeof(TC) + (N + 1) * 8;
+    static constexpr bool valid = N >= 1 && N <= 10 && lds <= 64 * 1024;
+};
+
+template <int N, int F, typename TI, typename TC, typename TA>
+__global__ __launch_bounds__(kLiftR) void k_lift4r(const TI *__restrict__ x, TI *__restrict__ out, int64_t K) {
+    constexpr int NP = np3(N), NFP = nfp2(N), MI = (NFP + 63) / 64, RS = LiftRCfg<N, F, TI, TC>::RS;
+    const LiftTab<N> &tb = kLiftTab<N>;
+    __shared__ __align__(16) TC S[4 * NP * RS];   // [face][slab node][field]
+    __shared__ __align__(16) TC X[4 * NFP * RS];  // [face][face node][field]
+    __shared__ double lw[N + 1];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);  // this wave's face
+    const int64_t e = blockIdx.x;
+    // per-lane table entries, read first (L2-resident)
+    int l0ab[MI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) {
+        const int beta = mi * 64 + lane;
+        l0ab[mi] = beta < NFP ? tb.ab[np3(N - 1) + beta] : 0;
+    }
+    const bool lact = lane < nfp2(N - 1);
+    const int chab = lact ? tb.ab[np3(N - 2) + lane] : 0;
+    // face w's data of every field -> records
+    const TI *xe = x + e * F * 4 * NFP + w * NFP;
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) {
+        const int m = mi * 64 + lane;
+        if (m < NFP) {
+            TC v[F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] = TC(xe[f * 4 * NFP + m]);
+            recv_store<F, RS>(X + __mul24(w * NFP + m, RS), v);
+        }
+    }
+    if (tid <= N) {
+#pragma unroll
+        for (int s = 0; s <= N; ++s)
+            if (tid == s) lw[s] = lw_chain(N, s);
+    }
+    wave_lds_sync();
+    // L0 (without its 1/2) -> slab 0 of chain w
+    const TC *xw = X + w * NFP * RS;
+    TC *sw = S + w * NP * RS;
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) {
+        const int beta = mi * 64 + lane;
+        if (beta < NFP) {
+            const L0Rec<TC> pt = l0_prep_rec<N, RS, TC>(l0ab[mi] & 255, l0ab[mi] >> 8);
+            TC v[F], y[F];
+            recv_load<F, RS>(xw + pt.o0, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] = pt.w0 * y[f];
+            recv_load<F, RS>(xw + pt.o1, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w1, y[f], v[f]);
+            recv_load<F, RS>(xw + pt.o2, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w2, y[f], v[f]);
+            recv_load<F, RS>(xw + pt.o3, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w3, y[f], v[f]);
+            recv_load<F, RS>(xw + pt.o4, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w4, y[f], v[f]);
+            recv_load<F, RS>(xw + pt.o5, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w5, y[f], v[f]);
+            recv_load<F, RS>(xw + pt.o6, y);
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] = fma(pt.w6, y[f], v[f]);
+            recv_store<F, RS>(sw + __mul24(beta, RS), v);
+        }
+    }
+    wave_lds_sync();
+    // the chain: (a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (d+1-a-b) x_{ab} (reduction_multiply!
+    // without /(d+1)), lane -> (a,b) fixed, offsets stepped by increments
+    {
+        const int a = chab & 255, b = chab >> 8;
+        const int bF = __mul24(b, RS);
+        int idxF = __mul24(lin2(N, a, b), RS), dstF = __mul24(lin2(N - 1, a, b), RS);
+        int o2F = idxF + __mul24(N + 1 - b, RS);
+        TC wc = TC(N - a - b);
+        const TC wa = TC(a + 1), wb = TC(b + 1);
+#pragma unroll
+        for (int s = 1; s <= N; ++s) {
+            if (lact && a + b <= N - s) {
+                const TC *src = sw + tet(N, s - 1) * RS;
+                TC x1[F], x2[F], x0[F], v[F];
+                recv_load<F, RS>(src + idxF + RS, x1);
+                recv_load<F, RS>(src + o2F, x2);
+                recv_load<F, RS>(src + idxF, x0);
+#pragma unroll
+                for (int f = 0; f < F; ++f) v[f] = fma(wa, x1[f], fma(wb, x2[f], wc * x0[f]));
+                recv_store<F, RS>(sw + tet(N, s) * RS + dstF, v);
+            }
+            wave_lds_sync();
+            idxF -= bF;
+            dstF -= bF;
+            o2F -= bF + RS;
+            wc -= TC(1);
+        }
+    }
+    __syncthreads();
+    // gather: out[n] = G4 + G1 + G2 + G3 with the slab weights
+    TI *oe = out + e * F * NP;
+    for (int n = tid; n < NP; n += kLiftR) {
+        const uint2 gp = reinterpret_cast<const uint2 *>(tb.go)[n];
+        const int go[4] = {(int)(gp.x & 0xffff), (int)(gp.x >> 16), (int)(gp.y & 0xffff), (int)(gp.y >> 16)};
+        const int sl = tb.sl[n];
+        const int slab[4] = {sl & 15, (sl >> 4) & 15, (sl >> 8) & 15, sl >> 12};
+        TA acc[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) acc[f] = TA(0);
+        constexpr int order[4] = {3, 0, 1, 2};
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int ff = order[o];
+            const TA wt = TA(lw[slab[ff]]);
+            TC v[F];
+            recv_load<F, RS>(S + ff * NP * RS + __mul24(go[ff], RS), v);
+#pragma unroll
+            for (int f = 0; f < F; ++f) acc[f] = fma(wt, TA(v[f]), acc[f]);
+        }
+#pragma unroll
+        for (int f = 0; f < F; ++f) oe[f * NP + n] = TI(acc[f]);
+    }
+}
+
+template <int N, typename TI, typename TC, typename TA>
+bool try_lift4r(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
+    constexpr int NP = np3(N), NFP = nfp2(N);
+    if (lx.elem != (int64_t)F * 4 * NFP || lx.node != 1 || (F > 1 && lx.field != 4 * NFP)) return false;
+    if (lo.elem != (int64_t)F * NP || lo.node != 1 || (F > 1 && lo.field != NP)) return false;
+    bool done = false;
+    auto one = [&](auto fc) {
+        constexpr int FF = decltype(fc)::value;
+        if constexpr (LiftRCfg<N, FF, TI, TC>::valid) {
+            if (F == FF) {
+                k_lift4r<N, FF, TI, TC, TA><<<(unsigned)K, kLiftR, 0, ctx->stream>>>((const TI *)x, (TI *)out, K);
+                done = true;
+            }
+        }
+    };
+    one(std::integral_constant<int, 1>{});
+    one(std::integral_constant<int, 2>{});
+    one(std::integral_constant<int, 3>{});
+    one(std::integral_constant<int, 4>{});
+    one(std::integral_constant<int, 6>{});
+    return done;
+}
+
 template <int N, typename TI, typename TC, typename TA>
 bool try_lift4w(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
     constexpr int NP = np3(N), NFP = nfp2(N);
@@ -401,7 +596,8 @@ __global__ __launch_bounds__(kLiftThreads) void k_face_lift(const TI *__restrict
 
 template <int N, typename TI, typename TC, typename TA>
 int launch_lift4(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
-    if (K <= 0x7fffffff && try_lift4w<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo)) {
+    if (K <= 0x7fffffff && (try_lift4r<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo) ||
+                            try_lift4w<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo))) {
         BB_LAUNCH_CHECK(ctx);
         return BB_OK;
     }

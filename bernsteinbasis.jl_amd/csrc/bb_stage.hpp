@@ -99,18 +99,19 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int F = Pde<PDE>::F;
     static constexpr int NP = np3(N), NFP = nfp2(N);
     static constexpr int NT = ((EB * NP + 63) / 64) * 64;
+    static constexpr int RS = rec_stride<T>(F);  // LDS record stride (values)
     // PINGPONG: s1 = state, then chain A; s2 = flux, then chain B
     // GATHER:   s1 = state, then all chain vectors (EB F 4 Np); s2 = flux
     static constexpr bool WAVEF = LM == LIFT_WAVE || LM == LIFT_WAVE_GT;
     static constexpr bool GATHER = LM == LIFT_GATHER || LM == LIFT_GATHER_MFMA || WAVEF;
-    static constexpr int R0 = WAVEF ? EB * F * NP : 0;  // separate state buffer (WAVE)
-    static constexpr int R1 = GATHER ? EB * F * 4 * NP
-                              : ((EB * F * NP > EB * F * 4 * NFP) ? EB * F * NP : EB * F * 4 * NFP);
-    static constexpr int R2 = EB * F * 4 * NFP;
+    static constexpr int R0 = WAVEF ? EB * RS * NP : 0;  // separate state buffer (WAVE)
+    static constexpr int R1 = GATHER ? EB * RS * 4 * NP
+                              : ((EB * RS * NP > EB * RS * 4 * NFP) ? EB * RS * NP : EB * RS * 4 * NFP);
+    static constexpr int R2 = EB * RS * 4 * NFP;
     static constexpr int TABG = (2 * NP + 28 * NFP + 3) / 4 * 4;  // ijk, dec, own / neighbour face nodes
     static constexpr int TAB = TABG + 4 * NP;                      // then the gather offsets
     static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
-    static constexpr int R3 = MD ? EB * 4 * F * kTraceMax : 0;  // neighbour traces of other degrees
+    static constexpr int R3 = MD ? EB * 4 * RS * kTraceMax : 0;  // neighbour traces of other degrees
     // index tables in LDS, or read from global memory (WAVE_GT)
     static constexpr bool TABL = LM != LIFT_WAVE_GT;
     static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (TABL ? (size_t)TAB16 * 16 : 0) + (N + 1) * 8;
@@ -155,16 +156,6 @@ inline std::vector<float> l0_mfma_table(int N) {
             }
     if (outside) t.clear();  // an entry the kernel would skip: refuse the table
     return t;
-}
-
-// Weight of slab s of the unnormalised lift chain of the stage kernel: slab 0 is 2 L0 x
-// and step t (degree N-t+1 -> N-t) omits the 1/(N-t+1) of reduction_multiply!, so slab s
-// is 2 N!/(N-s)! times the slab of face_lift_multiply! (src/BernsteinLift.jl:139-179);
-// its l_s (src/constants.jl:10-12) divided by that integer, rounded once
-__host__ __device__ constexpr double lw_chain(int N, int s) {
-    double d = 2.0;
-    for (int t = 0; t < s; ++t) d *= (N - t);
-    return lj(N, s) / d;
 }
 
 // Index tables of degree N for the stage kernel, built once on the host (uint16 each):
@@ -221,7 +212,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     using Cfg = StageCfg<N, PDE, MIXED, EB, LM, MD>;
     using T = typename Cfg::T;
     using P = Pde<PDE>;
-    constexpr int F = Cfg::F, NP = Cfg::NP, NFP = Cfg::NFP, NT = Cfg::NT;
+    constexpr int F = Cfg::F, NP = Cfg::NP, NFP = Cfg::NFP, NT = Cfg::NT, RS = Cfg::RS;
     // every LDS array holds node records of F values (field fastest): see recv_load
     __shared__ __align__(16) T s1[Cfg::R1];
     __shared__ __align__(16) T s2[Cfg::R2];
@@ -353,7 +344,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 T v[F];
 #pragma unroll
                 for (int f = 0; f < F; ++f) v[f] = T(qn[f * npn]);
-                recv_store<F>(s3 + __mul24((e2 * 4 + phi) * kTraceMax + jn, F), v);
+                recv_store<F, RS>(s3 + __mul24((e2 * 4 + phi) * kTraceMax + jn, RS), v);
             }
         }
     }
@@ -361,7 +352,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         T v[F];
 #pragma unroll
         for (int f = 0; f < F; ++f) v[f] = T(qv[f]);
-        recv_store<F>(sst + __mul24(el * NP + n, F), v);
+        recv_store<F, RS>(sst + __mul24(el * NP + n, RS), v);
     }
     __syncthreads();  // state visible
 
@@ -382,11 +373,11 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         for (int c = 0; c < 9; ++c) gh[c] = T(0.5 * g[c]);
         // fields in groups of G (one 16- or 8-byte word per stencil point and group)
         constexpr int G = (sizeof(T) == 4 && F % 4 == 0) ? 4 : 2;
-        const T *xs = sst + __mul24(el, NP * F);
+        const T *xs = sst + __mul24(el, NP * RS);
 #pragma unroll
         for (int g0 = 0; g0 < F; g0 += G) {
             T dr[G], ds[G], dt[G];
-            grad2_rec<N, F, G, T>(xs + g0, n, i, j, k, dr, ds, dt);
+            grad2_rec<N, RS, G, T>(xs + g0, n, i, j, k, dr, ds, dt);
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 const T gx = fma(gh[2], dt[q], fma(gh[1], ds[q], gh[0] * dr[q]));
@@ -407,7 +398,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         if (fitem(r, e2, phi, m)) {
             const int vi = Cfg::TABL ? fvid[phi * NFP + m] : fvp[r];
             T um[F], d[F], o[F];
-            recv_load<F>(sst + __mul24(e2 * NP + vi, F), um);
+            recv_load<F, RS>(sst + __mul24(e2 * NP + vi, RS), um);
             if (MD && ((cnv[r].y >> 5) & 7) == 0 && ((cnv[r].y >> 8) & 15) != N) {
                 // the neighbour's trace of degree Nn brought to degree N (in the neighbour's
                 // face orientation), read at this node through the face permutation
@@ -417,7 +408,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int q0 = perm_entry(p, 0), q1 = perm_entry(p, 1);
                 const int a2 = q0 == 0 ? a : (q0 == 1 ? b : c), b2 = q1 == 0 ? a : (q1 == 1 ? b : c);
                 const int mp = lin2(N, a2, b2);
-                const T *tn = s3 + __mul24(e2 * 4 + phi, kTraceMax * F);
+                const T *tn = s3 + __mul24(e2 * 4 + phi, kTraceMax * RS);
                 T up[F], tv[F];
 #pragma unroll
                 for (int f = 0; f < F; ++f) up[f] = T(0);
@@ -427,7 +418,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                     const int j1 = rp[1];
                     for (int jj = rp[0]; jj < j1; ++jj) {
                         const T w = MIXED ? T(A.evf[jj]) : T(A.evd[jj]);
-                        recv_load<F>(tn + __mul24(A.ecol[jj], F), tv);
+                        recv_load<F, RS>(tn + __mul24(A.ecol[jj], RS), tv);
 #pragma unroll
                         for (int f = 0; f < F; ++f) up[f] = fma(w, tv[f], up[f]);
                     }
@@ -436,7 +427,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                     const int64_t ro = A.troff[11 * N + Nn] + (int64_t)mp * nfn;
                     for (int jn = 0; jn < nfn; ++jn) {
                         const T w = MIXED ? T(A.trf[ro + jn]) : T(A.tr[ro + jn]);
-                        recv_load<F>(tn + __mul24(jn, F), tv);
+                        recv_load<F, RS>(tn + __mul24(jn, RS), tv);
 #pragma unroll
                         for (int f = 0; f < F; ++f) up[f] = fma(w, tv[f], up[f]);
                     }
@@ -452,7 +443,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             }
             const double *nj = A.nJ + ((e0 + e2) * 4 + phi) * 5;
             P::template flux<T>(A.flux_kind, d, T(nj[0]), T(nj[1]), T(nj[2]), T(nj[3]), T(nj[4]), o);
-            recv_store<F>(s2 + __mul24((e2 * 4 + phi) * NFP + m, F), o);
+            recv_store<F, RS>(s2 + __mul24((e2 * 4 + phi) * NFP + m, RS), o);
         }
     }
     if constexpr (Cfg::WAVEF) wave_lds_sync();
@@ -492,13 +483,13 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 #pragma unroll
             for (int nt = 0; nt < NTLM; ++nt) {
                 const int col = min(nt * 16 + (lane & 15), ncols - 1);
-                const T *xb = s2 + (col / F) * NFP * F + col % F;
+                const T *xb = s2 + (col / F) * NFP * RS + col % F;
 #pragma unroll
                 for (int kk = 0; kk < KT; ++kk) {
                     if (kk < klo || kk > khi) continue;
                     const int k = 4 * kk + (lane >> 4);
                     // rows of the padded k range meet zero columns of L0, but must be finite
-                    b[nt][kk] = (4 * kk + 3 < NFP || k < NFP) ? xb[(k < NFP ? k : NFP - 1) * F] : 0.f;
+                    b[nt][kk] = (4 * kk + 3 < NFP || k < NFP) ? xb[(k < NFP ? k : NFP - 1) * RS] : 0.f;
                 }
             }
             v4f acc[NTLM];
@@ -515,43 +506,43 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             for (int nt = 0; nt < NTLM; ++nt) {
                 const int col = nt * 16 + (lane & 15);
                 if (col < ncols) {
-                    T *yb = s1 + (col / F) * QS * F + col % F;
+                    T *yb = s1 + (col / F) * QS * RS + col % F;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int beta = 16 * mt + 4 * (lane >> 4) + r;
-                        if (beta < NFP) yb[beta * F] = acc[nt][r];
+                        if (beta < NFP) yb[beta * RS] = acc[nt][r];
                     }
                 }
             }
         }
     } else {
         auto l0_item = [&](int c, int beta, int ab) {
-            const L0Rec<T> pt = l0_prep_rec<N, F, T>(ab & 255, ab >> 8);
-            const T *x = s2 + __mul24(c, NFP * F);
+            const L0Rec<T> pt = l0_prep_rec<N, RS, T>(ab & 255, ab >> 8);
+            const T *x = s2 + __mul24(c, NFP * RS);
             T v[F], y[F];
             // the 7-point sum in the order of l0_eval, as explicit fma
-            recv_load<F>(x + pt.o0, y);
+            recv_load<F, RS>(x + pt.o0, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = pt.w0 * y[f];
-            recv_load<F>(x + pt.o1, y);
+            recv_load<F, RS>(x + pt.o1, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w1, y[f], v[f]);
-            recv_load<F>(x + pt.o2, y);
+            recv_load<F, RS>(x + pt.o2, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w2, y[f], v[f]);
-            recv_load<F>(x + pt.o3, y);
+            recv_load<F, RS>(x + pt.o3, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w3, y[f], v[f]);
-            recv_load<F>(x + pt.o4, y);
+            recv_load<F, RS>(x + pt.o4, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w4, y[f], v[f]);
-            recv_load<F>(x + pt.o5, y);
+            recv_load<F, RS>(x + pt.o5, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w5, y[f], v[f]);
-            recv_load<F>(x + pt.o6, y);
+            recv_load<F, RS>(x + pt.o6, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w6, y[f], v[f]);
-            recv_store<F>(s1 + __mul24(c * QS + beta, F), v);
+            recv_store<F, RS>(s1 + __mul24(c * QS + beta, RS), v);
         };
         if constexpr (Cfg::WAVEF) {
             for (int q = 0; q < CW; ++q) {
@@ -572,15 +563,15 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     // one unnormalised reduction step, degree dd+1 -> dd, at node (a,b) of chain c:
     //   (a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (dd+1-a-b) x_{ab}   (reduction_multiply! without /(dd+1))
     // src points at the chain's source slab, dst at the output record; record offsets
-    // idxF = F lin2(dd+1, a, b) and o2F = F (lin2(dd+1, a, b) + dd + 2 - b)
+    // idxF = RS lin2(dd+1, a, b) and o2F = RS (lin2(dd+1, a, b) + dd + 2 - b)
     auto chain_item = [&](const T *src, T *dst, int idxF, int o2F, T wa, T wb, T wc) {
         T x1[F], x2[F], x0[F], v[F];
-        recv_load<F>(src + idxF + F, x1);
-        recv_load<F>(src + o2F, x2);
-        recv_load<F>(src + idxF, x0);
+        recv_load<F, RS>(src + idxF + RS, x1);
+        recv_load<F, RS>(src + o2F, x2);
+        recv_load<F, RS>(src + idxF, x0);
 #pragma unroll
         for (int f = 0; f < F; ++f) v[f] = fma(wa, x1[f], fma(wb, x2[f], wc * x0[f]));
-        recv_store<F>(dst, v);
+        recv_store<F, RS>(dst, v);
     };
 
     if constexpr (Cfg::WAVEF) {
@@ -595,9 +586,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             a = ab & 255;
             b = ab >> 8;
         }
-        const int bF = __mul24(b, F);
-        int idxF = __mul24(lin2(N, a, b), F), dstF = __mul24(lin2(N - 1, a, b), F);
-        int o2F = idxF + __mul24(N + 1 - b, F);
+        const int bF = __mul24(b, RS);
+        int idxF = __mul24(lin2(N, a, b), RS), dstF = __mul24(lin2(N - 1, a, b), RS);
+        int o2F = idxF + __mul24(N + 1 - b, RS);
         T wc = T(N - a - b);
         const T wa = T(a + 1), wb = T(b + 1);
 #pragma unroll
@@ -607,14 +598,14 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 for (int q = 0; q < CW; ++q) {
                     const int c = wave_u + NW * q;
                     if (c >= ne * 4) break;
-                    chain_item(s1 + (c * NP + tet(N, s - 1)) * F, s1 + (c * NP + tet(N, s)) * F + dstF, idxF, o2F,
+                    chain_item(s1 + (c * NP + tet(N, s - 1)) * RS, s1 + (c * NP + tet(N, s)) * RS + dstF, idxF, o2F,
                                wa, wb, wc);
                 }
             }
             wave_lds_sync();
             idxF -= bF;
             dstF -= bF;
-            o2F -= bF + F;
+            o2F -= bF + RS;
             wc -= T(1);
         }
         __syncthreads();  // every chain complete before the gather
@@ -625,9 +616,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int c = t / nf, beta = t % nf;
                 const int ab = dec[np3(dd - 1) + beta];
                 const int a = ab & 255, b = ab >> 8;
-                const int idxF = __mul24(lin2(dd + 1, a, b), F);
-                chain_item(src + __mul24(c * QS + src_off, F), dst + __mul24(c * QS + dst_off + beta, F), idxF,
-                           idxF + __mul24(dd + 2 - b, F), T(a + 1), T(b + 1), T(dd + 1 - a - b));
+                const int idxF = __mul24(lin2(dd + 1, a, b), RS);
+                chain_item(src + __mul24(c * QS + src_off, RS), dst + __mul24(c * QS + dst_off + beta, RS), idxF,
+                           idxF + __mul24(dd + 2 - b, RS), T(a + 1), T(b + 1), T(dd + 1 - a - b));
             }
         };
         if constexpr (Cfg::GATHER) {
@@ -647,7 +638,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int ff = order[o];
                 const double w = lw[st[ff]];
                 T v[F];
-                recv_load<F>(s1 + __mul24(el * 4 + ff, NP * F) + __mul24(go[ff], F), v);
+                recv_load<F, RS>(s1 + __mul24(el * 4 + ff, NP * RS) + __mul24(go[ff], RS), v);
 #pragma unroll
                 for (int f = 0; f < F; ++f) acc[f] = fma(w, (double)v[f], acc[f]);
             }
@@ -661,7 +652,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int ff = order[o];
                 if (st[ff] == s) {
                     T v[F];
-                    recv_load<F>(buf + __mul24((el * 4 + ff) * NFP + ix[ff], F), v);
+                    recv_load<F, RS>(buf + __mul24((el * 4 + ff) * NFP + ix[ff], RS), v);
 #pragma unroll
                     for (int f = 0; f < F; ++f) acc[f] = fma(w, (double)v[f], acc[f]);
                 }
@@ -677,9 +668,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 const int c = t / nf, beta = t % nf;
                 const int ab = dec[np3(dd - 1) + beta];
                 const int a = ab & 255, b = ab >> 8;
-                const int idxF = __mul24(lin2(dd + 1, a, b), F);
-                chain_item(cur + __mul24(c, NFP * F), nxt + __mul24(c * NFP + beta, F), idxF,
-                           idxF + __mul24(dd + 2 - b, F), T(a + 1), T(b + 1), T(dd + 1 - a - b));
+                const int idxF = __mul24(lin2(dd + 1, a, b), RS);
+                chain_item(cur + __mul24(c, NFP * RS), nxt + __mul24(c * NFP + beta, RS), idxF,
+                           idxF + __mul24(dd + 2 - b, RS), T(a + 1), T(b + 1), T(dd + 1 - a - b));
             }
             __syncthreads();
             accumulate(s, nxt);

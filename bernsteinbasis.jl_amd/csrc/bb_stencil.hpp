@@ -148,11 +148,21 @@ template <typename T> __device__ __forceinline__ T l0_eval(const T *x, const L0P
     return T(0.5) * s;
 }
 
-// The F fields of one node stored next to each other (LDS records of F values): one
-// address per node, read and written as 16-byte words where the record size is a multiple
-// of 16 bytes, else as 8-byte words (p aligned to that word size)
-template <int F, typename T> __device__ __forceinline__ void recv_load(const T *p, T (&v)[F]) {
-    constexpr int W = (F * (int)sizeof(T)) % 16 == 0 ? 16 / (int)sizeof(T) : 8 / (int)sizeof(T);
+// The F fields of one node stored next to each other (LDS records of F values, RS values
+// apart): one address per node, read and written as 16-byte words where the record stride
+// and the access are multiples of 16 bytes, else as 8-byte words where they are multiples
+// of 8, else value by value (p is then aligned to that word size)
+template <int F, int RS, typename T> constexpr int recv_word() {
+    constexpr int B = RS * (int)sizeof(T), A = F * (int)sizeof(T);
+    return (B % 16 == 0 && A % 16 == 0) ? 16 / (int)sizeof(T) : (B % 8 == 0 && A % 8 == 0) ? 8 / (int)sizeof(T) : 1;
+}
+// Record stride for F values of type T.  Records of 32 bytes (four FP64 fields) meet
+// two-way LDS bank conflicts under 16-byte accesses; padding them to 40 bytes removes the
+// conflicts but halves the access width, and measured slower (the 4-face lift at N=7:
+// 1087 -> 1350 us), so records are not padded.
+template <typename T> constexpr int rec_stride(int F) { return F; }
+template <int F, int RS = F, typename T> __device__ __forceinline__ void recv_load(const T *p, T (&v)[F]) {
+    constexpr int W = recv_word<F, RS, T>();
     if constexpr (W == 1 || F % W != 0) {
 #pragma unroll
         for (int f = 0; f < F; ++f) v[f] = p[f];
@@ -167,8 +177,8 @@ template <int F, typename T> __device__ __forceinline__ void recv_load(const T *
         }
     }
 }
-template <int F, typename T> __device__ __forceinline__ void recv_store(T *p, const T (&v)[F]) {
-    constexpr int W = (F * (int)sizeof(T)) % 16 == 0 ? 16 / (int)sizeof(T) : 8 / (int)sizeof(T);
+template <int F, int RS = F, typename T> __device__ __forceinline__ void recv_store(T *p, const T (&v)[F]) {
+    constexpr int W = recv_word<F, RS, T>();
     if constexpr (W == 1 || F % W != 0) {
 #pragma unroll
         for (int f = 0; f < F; ++f) p[f] = v[f];
@@ -185,8 +195,8 @@ template <int F, typename T> __device__ __forceinline__ void recv_store(T *p, co
     }
 }
 
-// Twice the three reference-coordinate derivatives of G fields of node records of F
-// values (x + F m holds node m; x points at the group's first field): the 13 stencil
+// Twice the three reference-coordinate derivatives of G fields of node records F values
+// apart (x + F m holds node m; x points at the group's first field): the 13 stencil
 // points of grad_ref_ld, each read once for the group, the four weighted differences per
 // derivative in the same order (as explicit fma, so that every kernel rounds alike),
 // without the factor 1/2 (the caller folds it into the geometric factors)
@@ -202,32 +212,32 @@ __device__ __forceinline__ void grad2_rec(const T *x, int idx, int i, int j, int
     const int kmF = cF + __mul24(j - skm, F), kpF = cF + __mul24(sk - j, F);
     const T ti = T(i), tj = T(j), tk = T(k), tl = T(l);
     T xc[G], xim[G], xjm[G], xkm[G], y[G], z[G];
-    recv_load<G>(x + cF, xc);
-    recv_load<G>(x + (i > 0 ? cF - F : cF), xim);
-    recv_load<G>(x + (j > 0 ? cF - rlF - F : cF), xjm);
-    recv_load<G>(x + (k > 0 ? kmF : cF), xkm);
+    recv_load<G, F>(x + cF, xc);
+    recv_load<G, F>(x + (i > 0 ? cF - F : cF), xim);
+    recv_load<G, F>(x + (j > 0 ? cF - rlF - F : cF), xjm);
+    recv_load<G, F>(x + (k > 0 ? kmF : cF), xkm);
     // d/dr: i (x_c - x_{i-1}) + j (x_{i+1,j-1} - x_{j-1}) + k (x_{i+1,k-1} - x_{k-1}) + l (x_{i+1} - x_c)
-    recv_load<G>(x + (j > 0 ? cF - rlF : cF), y);
-    recv_load<G>(x + (k > 0 ? kmF + F : cF), z);
+    recv_load<G, F>(x + (j > 0 ? cF - rlF : cF), y);
+    recv_load<G, F>(x + (k > 0 ? kmF + F : cF), z);
 #pragma unroll
     for (int f = 0; f < G; ++f) dr[f] = fma(tk, z[f] - xkm[f], fma(tj, y[f] - xjm[f], ti * (xc[f] - xim[f])));
-    recv_load<G>(x + (l > 0 ? cF + F : cF), y);
+    recv_load<G, F>(x + (l > 0 ? cF + F : cF), y);
 #pragma unroll
     for (int f = 0; f < G; ++f) dr[f] = fma(tl, y[f] - xc[f], dr[f]);
     // d/ds: i (x_{i-1,j+1} - x_{i-1}) + j (x_c - x_{j-1}) + k (x_{j+1,k-1} - x_{k-1}) + l (x_{j+1} - x_c)
-    recv_load<G>(x + (i > 0 ? cF + rlF - F : cF), y);
-    recv_load<G>(x + (k > 0 ? kmF + rlF + F : cF), z);
+    recv_load<G, F>(x + (i > 0 ? cF + rlF - F : cF), y);
+    recv_load<G, F>(x + (k > 0 ? kmF + rlF + F : cF), z);
 #pragma unroll
     for (int f = 0; f < G; ++f) ds[f] = fma(tk, z[f] - xkm[f], fma(tj, xc[f] - xjm[f], ti * (y[f] - xim[f])));
-    recv_load<G>(x + (l > 0 ? cF + rlF : cF), y);
+    recv_load<G, F>(x + (l > 0 ? cF + rlF : cF), y);
 #pragma unroll
     for (int f = 0; f < G; ++f) ds[f] = fma(tl, y[f] - xc[f], ds[f]);
     // d/dt: i (x_{i-1,k+1} - x_{i-1}) + j (x_{j-1,k+1} - x_{j-1}) + k (x_c - x_{k-1}) + l (x_{k+1} - x_c)
-    recv_load<G>(x + (i > 0 ? kpF - F : cF), y);
-    recv_load<G>(x + (j > 0 ? kpF - rlF : cF), z);
+    recv_load<G, F>(x + (i > 0 ? kpF - F : cF), y);
+    recv_load<G, F>(x + (j > 0 ? kpF - rlF : cF), z);
 #pragma unroll
     for (int f = 0; f < G; ++f) dt[f] = fma(tk, xc[f] - xkm[f], fma(tj, z[f] - xjm[f], ti * (y[f] - xim[f])));
-    recv_load<G>(x + (l > 0 ? kpF : cF), y);
+    recv_load<G, F>(x + (l > 0 ? kpF : cF), y);
 #pragma unroll
     for (int f = 0; f < G; ++f) dt[f] = fma(tl, y[f] - xc[f], dt[f]);
 }
@@ -257,6 +267,16 @@ template <int N, int F, typename T> __device__ __forceinline__ L0Rec<T> l0_prep_
     p.w5 = T((c + 1) * a);
     p.w6 = T((c + 1) * b);
     return p;
+}
+
+// Weight of slab s of the unnormalised lift chain (stage and lift kernels): slab 0 is 2 L0 x
+// and step t (degree N-t+1 -> N-t) omits the 1/(N-t+1) of reduction_multiply!, so slab s
+// is 2 N!/(N-s)! times the slab of face_lift_multiply! (src/BernsteinLift.jl:139-179);
+// its l_s (src/constants.jl:10-12) divided by that integer, rounded once
+__host__ __device__ constexpr double lw_chain(int N, int s) {
+    double d = 2.0;
+    for (int t = 0; t < s; ++t) d *= (N - t);
+    return lj(N, s) / d;
 }
 
 // (E_{d}^{d+1})^T x: degree d+1 -> degree d at (a,b), reduction_multiply!
