@@ -293,7 +293,7 @@ template <int N> struct LiftTab {
 };
 template <int N> __device__ constexpr LiftTab<N> kLiftTab = LiftTab<N>();
 
-// Compact field-major layouts with F (1..4, 6) as a template parameter and N <= 10: one
+// Compact field-major layouts with F (1..4, 6) as a template parameter and 7 <= N <= 10: one
 // block of four waves per element, wave f owning face f's chain for all F fields.  The
 // face data become LDS records of F values (field fastest), so that a stencil point is one
 // address and one or two vector accesses for all fields.  The chain is unnormalised (slab
@@ -307,7 +307,9 @@ template <int N, int F, typename TI, typename TC> struct LiftRCfg {
     static constexpr int NP = np3(N), NFP = nfp2(N);
     static constexpr int RS = rec_stride<TC>(F);  // LDS record stride (values)
     static constexpr size_t lds = (size_t)4 * (NP + NFP) * RS * sizeof(TC) + (N + 1) * 8;
-    static constexpr bool valid = N >= 1 && N <= 10 && lds <= 64 * 1024;
+    // below N = 7 the four waves per element leave most lanes idle and the two-wave kernel
+    // (k_lift4w) is faster (measured at N = 3, 5: 1.5x)
+    static constexpr bool valid = N >= 7 && N <= 10 && lds <= 64 * 1024;
 };
 
 template <int N, int F, typename TI, typename TC, typename TA>
