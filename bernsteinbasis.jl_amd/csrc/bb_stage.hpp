@@ -416,6 +416,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                     // elevation: the few neighbour nodes below this one (sparse row)
                     const int32_t *rp = A.erow + A.eoff[11 * N + Nn] + mp;
                     const int j1 = rp[1];
+#pragma unroll 4
                     for (int jj = rp[0]; jj < j1; ++jj) {
                         const T w = MIXED ? T(A.evf[jj]) : T(A.evd[jj]);
                         recv_load<F, RS>(tn + __mul24(A.ecol[jj], RS), tv);
@@ -425,6 +426,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 } else {
                     // L2 projection from a higher degree: a dense row
                     const int64_t ro = A.troff[11 * N + Nn] + (int64_t)mp * nfn;
+#pragma unroll 8
                     for (int jn = 0; jn < nfn; ++jn) {
                         const T w = MIXED ? T(A.trf[ro + jn]) : T(A.tr[ro + jn]);
                         recv_load<F, RS>(tn + __mul24(jn, RS), tv);
