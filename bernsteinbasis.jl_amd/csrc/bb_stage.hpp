@@ -234,7 +234,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     const bool own = tid < ne * NP;
     const int el = own ? tid / NP : 0;
     const int n = own ? tid % NP : 0;
-    const int64_t eg = e0 + el;
+    // with one element per workgroup its index is uniform, so per-element data (1/J, the
+    // geometric factors) can be read with scalar loads
+    const int64_t eg = EB == 1 ? e0 : e0 + el;
 
     // ---- prologue: every global read the stage needs is issued here, before the first
     // LDS work, so that its latency overlaps the volume phase:
@@ -269,12 +271,15 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     TR *res = reinterpret_cast<TR *>(A.res);
     double qv[F];
     TR rv[F];
+    double invJ = 0.0, dte = 0.0;  // read here: a load issued in the epilogue stalls the workgroup's end
     if (own) {
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             qv[f] = A.q_in[(eg * F + f) * NP + n];
             rv[f] = (A.first || A.rhs_only) ? TR(0) : res[(eg * F + f) * NP + n];
         }
+        invJ = A.invJ[eg];
+        dte = MD ? A.dt * A.dte[eg] : A.dt;
     }
     __syncthreads();  // tables visible
     // face-node work items of this thread (at most RF of them).  WAVE: chain c = (element,
@@ -684,7 +689,6 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 
     // ---- epilogue: rhs = (lift + volume) / J, then the RK stage
     if (own) {
-        const double invJ = A.invJ[eg];
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             const double rhs = (acc[f] + (double)vol[f]) * invJ;
@@ -692,7 +696,6 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             if (A.rhs_only) {
                 res[g] = TR(rhs);
             } else {
-                const double dte = MD ? A.dt * A.dte[eg] : A.dt;
                 const double r = A.first ? dte * rhs : fma(A.ca, (double)rv[f], dte * rhs);
                 const TR rs = TR(r);
                 res[g] = rs;
