@@ -101,7 +101,7 @@ class Solver:
     @staticmethod
     def tiling_code(elems_per_block, lift="gather"):
         return int(elems_per_block) + 16 * {"gather": 0, "pingpong": 1, "gather_mfma": 2, "wave": 3,
-                                            "wave_gt": 4}[lift]
+                                            "wave_gt": 4, "wave1": 5}[lift]
 
     def stage_bytes(self, prec="f64"):
         b, d = C.c_double(), C.c_double()

@@ -32,6 +32,7 @@ struct bb_batch {
     double sign[6] = {1, 1, 1, 1, 1, 1};
     std::vector<BatchGroup> groups;
     double *geo = nullptr, *invJ = nullptr, *nJ = nullptr, *dte = nullptr, *tr = nullptr, *evd = nullptr;
+    float *geo32 = nullptr;  // geo / 2 in FP32 (mixed precision); geo holds geo / 2 in FP64
     float *trf = nullptr, *evf = nullptr;
     int32_t *erow = nullptr;
     uint8_t *ecol = nullptr;
@@ -61,7 +62,7 @@ static StageArgs batch_args(bb_batch *b, const BatchGroup &g, int prec, int cur)
     a.q_in = b->q[cur] + g.qoff;
     a.q_out = b->q[cur ^ 1] + g.qoff;
     a.res = prec == BB_PREC_MIXED ? (void *)(b->res32 + g.qoff) : (void *)(b->res64 + g.qoff);
-    a.geo = b->geo + g.start * 9;
+    a.geoh = prec == BB_PREC_MIXED ? (const void *)(b->geo32 + g.start * 9) : (const void *)(b->geo + g.start * 9);
     a.invJ = b->invJ + g.start;
     a.nJ = b->nJ + g.start * 20;
     a.conn = b->conn + g.start * 4;
@@ -93,7 +94,7 @@ static int batch_launch(bb_batch *b, const BatchGroup &g, int prec, int tiling, 
 
 static int batch_default_tiling(int prec, int N) {
     const int NP = np3(N);
-    for (int lm : {LIFT_GATHER, LIFT_PINGPONG, LIFT_WAVE, LIFT_WAVE_GT}) {
+    for (int lm : {LIFT_GATHER, LIFT_PINGPONG, LIFT_WAVE, LIFT_WAVE_GT, LIFT_WAVE1}) {
         int best = 0;
         for (int e : kStageEB) {
             if (e * NP > 1024) break;
@@ -252,9 +253,15 @@ int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_
         return r;
     };
     hipError_t err;
-    std::vector<double> invJ(K);
+    std::vector<double> invJ(K), geoh(K * 9);
+    std::vector<float> geoh32(K * 9);
     for (int64_t e = 0; e < K; ++e) invJ[e] = 1.0 / J_host[e];
+    for (int64_t t = 0; t < K * 9; ++t) {
+        geoh[t] = 0.5 * geo_host[t];
+        geoh32[t] = (float)geoh[t];
+    }
     if ((err = hipMalloc(&b->geo, K * 9 * sizeof(double))) || (err = hipMalloc(&b->invJ, K * sizeof(double))) ||
+        (err = hipMalloc(&b->geo32, K * 9 * sizeof(float))) ||
         (err = hipMalloc(&b->nJ, K * 20 * sizeof(double))) || (err = hipMalloc(&b->dte, K * sizeof(double))) ||
         (err = hipMalloc(&b->conn, K * 4 * sizeof(int2))) || (err = hipMalloc(&b->nboff, K * 4 * sizeof(int64_t))) ||
         (err = hipMalloc(&b->tr, (transfer_len > 0 ? transfer_len : 1) * sizeof(double))) ||
@@ -266,7 +273,8 @@ int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_
         (err = hipMalloc(&b->evf, (evf.size() + 1) * sizeof(float))) ||
         (err = hipMalloc(&b->q[1], words * sizeof(double))))
         return fail(err);
-    if ((err = hipMemcpy(b->geo, geo_host, K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
+    if ((err = hipMemcpy(b->geo, geoh.data(), K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->geo32, geoh32.data(), K * 9 * sizeof(float), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->invJ, invJ.data(), K * sizeof(double), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->nJ, nj5.data(), K * 20 * sizeof(double), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->dte, dt_host, K * sizeof(double), hipMemcpyHostToDevice)) ||
@@ -304,7 +312,7 @@ int bb_batch_destroy(bb_batch *b) {
     if (!b) return BB_ERR_NULL;
     hipStreamSynchronize(b->ctx->stream);
     batch_drop_graphs(b);
-    void *ptrs[] = {b->geo, b->invJ, b->nJ, b->dte, b->tr, b->conn, b->nboff, b->troff, b->q[0], b->q[1], b->res64, b->res32,
+    void *ptrs[] = {b->geo, b->geo32, b->invJ, b->nJ, b->dte, b->tr, b->conn, b->nboff, b->troff, b->q[0], b->q[1], b->res64, b->res32,
                     b->trf, b->erow, b->eoff, b->ecol, b->evd, b->evf};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -436,7 +444,7 @@ int bb_batch_autotune(bb_batch *b, int prec, int *tilings) {
         a.first = 0;
         float best = 1e30f;
         int best_t = 0;
-        for (int lm : {LIFT_GATHER, LIFT_PINGPONG, LIFT_WAVE, LIFT_WAVE_GT})
+        for (int lm : {LIFT_GATHER, LIFT_PINGPONG, LIFT_WAVE, LIFT_WAVE_GT, LIFT_WAVE1})
             for (int e : kStageEB) {
                 const int t = tiling_code(e, lm);
                 if (!batch_valid(prec, g.N, t)) continue;

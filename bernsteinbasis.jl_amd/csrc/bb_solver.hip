@@ -18,6 +18,8 @@ struct bb_solver {
     double sign[6] = {1, 1, 1, 1, 1, 1};
     // mesh
     double *geo = nullptr, *J = nullptr, *invJ = nullptr, *nJ = nullptr;  // nJ: [K][4][5] nxJ nyJ nzJ Jf 1/Jf
+    double *geoh64 = nullptr;  // geo / 2, the fused kernels' operands (FP64 and FP32)
+    float *geoh32 = nullptr;
     int2 *conn = nullptr;
     uint16_t *tab = nullptr;
     float *l0a = nullptr;  // L0 as MFMA A operands (mixed precision, matrix-core L0 tilings)
@@ -96,7 +98,7 @@ static int launch_stage(bb_solver *s, int prec, int tiling, const StageArgs &a) 
 
 static StageArgs stage_args(bb_solver *s, int prec) {
     StageArgs a{};
-    a.geo = s->geo;
+    a.geoh = prec == BB_PREC_MIXED ? (const void *)s->geoh32 : (const void *)s->geoh64;
     a.invJ = s->invJ;
     a.nJ = s->nJ;
     a.conn = s->conn;
@@ -295,10 +297,16 @@ int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, cons
         delete s;
         return bb_fail(ctx, BB_ERR_STATE, "L0 has entries outside the band the matrix-core kernel assumes (N=%d)", N);
     }
-    std::vector<double> invJ(K);
+    std::vector<double> invJ(K), geoh64(K * 9);
+    std::vector<float> geoh32(K * 9);
     for (int64_t e2 = 0; e2 < K; ++e2) invJ[e2] = 1.0 / J_host[e2];
+    for (int64_t t = 0; t < K * 9; ++t) {
+        geoh64[t] = 0.5 * geo_host[t];
+        geoh32[t] = (float)geoh64[t];
+    }
     if ((e = hipMalloc(&s->geo, K * 9 * sizeof(double))) || (e = hipMalloc(&s->J, K * sizeof(double))) ||
-        (e = hipMalloc(&s->invJ, K * sizeof(double))) ||
+        (e = hipMalloc(&s->invJ, K * sizeof(double))) || (e = hipMalloc(&s->geoh64, K * 9 * sizeof(double))) ||
+        (e = hipMalloc(&s->geoh32, K * 9 * sizeof(float))) ||
         (e = hipMalloc(&s->nJ, K * 20 * sizeof(double))) || (e = hipMalloc(&s->conn, K * 4 * sizeof(int2))) ||
         (e = hipMalloc(&s->tab, tab.size() * sizeof(uint16_t))) ||
         (e = hipMalloc(&s->l0a, l0a.size() * sizeof(float))) ||
@@ -307,6 +315,8 @@ int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, cons
     if ((e = hipMemcpy(s->geo, geo_host, K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->J, J_host, K * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->invJ, invJ.data(), K * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->geoh64, geoh64.data(), K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->geoh32, geoh32.data(), K * 9 * sizeof(float), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->nJ, nj5.data(), K * 20 * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->conn, conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->tab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice)) ||
@@ -324,7 +334,7 @@ int bb_solver_destroy(bb_solver *s) {
     if (!s) return BB_ERR_NULL;
     hipStreamSynchronize(s->ctx->stream);
     drop_graphs(s);
-    void *ptrs[] = {s->geo, s->J, s->invJ, s->nJ, s->conn, s->tab, s->l0a, s->mapP, s->bcn, s->nx, s->ny, s->nz, s->uM, s->flux, s->du,
+    void *ptrs[] = {s->geo, s->geoh64, s->geoh32, s->J, s->invJ, s->nJ, s->conn, s->tab, s->l0a, s->mapP, s->bcn, s->nx, s->ny, s->nz, s->uM, s->flux, s->du,
                     s->q[0], s->q[1], s->res64, s->res32};
     for (void *p : ptrs)
         if (p) hipFree(p);

@@ -8,7 +8,7 @@
 //
 // Data in HBM (per element, all contiguous, element-major):
 //   q_in, q_out [K][F][Np] FP64 (ping-pong: neighbours read the old stage while the new
-//   one is written), res [K][F][Np] (FP64, or FP32 in mixed precision), geo [K][9],
+//   one is written), res [K][F][Np] (FP64, or FP32 in mixed precision), geo / 2 [K][9],
 //   1/J [K], nJ [K][4][5], conn [K][4] = {neighbour, face | perm << 2 | bc << 5}.
 //
 // LDS (T = FP64, or FP32 in mixed precision), every array as node records of F values:
@@ -30,7 +30,7 @@ struct StageArgs {
     const double *q_in;
     double *q_out;
     void *res;
-    const double *geo;
+    const void *geoh;      // [K][9] halved geometric factors rxJ..tzJ / 2 in the kernel's type
     const double *invJ;    // [K] 1/J
     const double *nJ;      // [K][4][5]: nxJ nyJ nzJ Jf 1/Jf per face
     const int2 *conn;
@@ -91,7 +91,11 @@ constexpr int kTraceMax = 66;  // face nodes of the highest degree of a fused ke
 //             entry is read once per workgroup, and the chain steps by increments), which
 //             leaves the LDS to the state and chains and admits more workgroups per CU
 //             where the LDS limits them.
-enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1, LIFT_GATHER_MFMA = 2, LIFT_WAVE = 3, LIFT_WAVE_GT = 4 };
+//  WAVE1    : WAVE_GT, with the chains' later steps packed: once a step's output has at
+//             most 32 nodes (degree <= 6), a workgroup barrier hands the chains over and each
+//             wave runs two chains in the two halves of its lanes (a wave-owned chain keeps
+//             most lanes idle in those steps: 36 wave-steps per element at N = 9, 22 here).
+enum { LIFT_GATHER = 0, LIFT_PINGPONG = 1, LIFT_GATHER_MFMA = 2, LIFT_WAVE = 3, LIFT_WAVE_GT = 4, LIFT_WAVE1 = 5 };
 
 
 template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = false> struct StageCfg {
@@ -102,7 +106,7 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int RS = rec_stride<T>(F);  // LDS record stride (values)
     // PINGPONG: s1 = state, then chain A; s2 = flux, then chain B
     // GATHER:   s1 = state, then all chain vectors (EB F 4 Np); s2 = flux
-    static constexpr bool WAVEF = LM == LIFT_WAVE || LM == LIFT_WAVE_GT;
+    static constexpr bool WAVEF = LM == LIFT_WAVE || LM == LIFT_WAVE_GT || LM == LIFT_WAVE1;
     static constexpr bool GATHER = LM == LIFT_GATHER || LM == LIFT_GATHER_MFMA || WAVEF;
     static constexpr int R0 = WAVEF ? EB * RS * NP : 0;  // separate state buffer (WAVE)
     static constexpr int R1 = GATHER ? EB * RS * 4 * NP
@@ -113,7 +117,7 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
     static constexpr int R3 = MD ? EB * 4 * RS * kTraceMax : 0;  // neighbour traces of other degrees
     // index tables in LDS, or read from global memory (WAVE_GT)
-    static constexpr bool TABL = LM != LIFT_WAVE_GT;
+    static constexpr bool TABL = LM != LIFT_WAVE_GT && LM != LIFT_WAVE1;
     static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (TABL ? (size_t)TAB16 * 16 : 0) + (N + 1) * 8;
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
     static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
@@ -220,6 +224,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     T *sst = Cfg::WAVEF ? s0 : s1;  // the element state in LDS: [element][node][field]
     __shared__ __align__(16) T s3[MD ? Cfg::R3 : 1];
     __shared__ uint4 tab16[Cfg::TABL ? Cfg::TAB16 : 1];
+
     const uint16_t *tab = Cfg::TABL ? reinterpret_cast<const uint16_t *>(tab16) : A.tab;
     __shared__ double lw[N + 1];  // slab weights of the unnormalised chain (lw_chain)
     const uint16_t *ijk = tab, *dec = tab + NP, *fvid = tab + 2 * NP, *nvid = tab + 2 * NP + 4 * NFP;
@@ -371,11 +376,13 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     }
     const int i = pk & 15, j = (pk >> 4) & 15, k = pk >> 8;
     if (own) {
-        // halved geometric factors: grad2_rec returns twice the reference derivatives
-        const double *g = A.geo + eg * 9;
+        // halved geometric factors (grad2_rec returns twice the reference derivatives),
+        // prepared on the host in T: with one element per workgroup they are scalar loads
+        // that the FMAs read from scalar registers
+        const T *g = reinterpret_cast<const T *>(A.geoh) + eg * 9;
         T gh[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) gh[c] = T(0.5 * g[c]);
+        for (int c = 0; c < 9; ++c) gh[c] = g[c];
         // fields in groups of G (one 16- or 8-byte word per stencil point and group)
         constexpr int G = (sizeof(T) == 4 && F % 4 == 0) ? 4 : 2;
         const T *xs = sst + __mul24(el, NP * RS);
@@ -598,8 +605,10 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         int o2F = idxF + __mul24(N + 1 - b, RS);
         T wc = T(N - a - b);
         const T wa = T(a + 1), wb = T(b + 1);
+        // WAVE1: steps down to output degree DS + 1 as above, the rest two chains per wave
+        constexpr int DS = LM == LIFT_WAVE1 ? (N - 1 < 6 ? N - 1 : 6) : -1;
 #pragma unroll
-        for (int s = 1; s <= N; ++s) {
+        for (int s = 1; s <= N - 1 - DS; ++s) {
             const int dd = N - s;
             if (lact && a + b <= dd) {
                 for (int q = 0; q < CW; ++q) {
@@ -614,6 +623,43 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             dstF -= bF;
             o2F -= bF + RS;
             wc -= T(1);
+        }
+        if constexpr (DS >= 0) {
+            __syncthreads();  // hand-over: every chain at degree DS + 1
+            // lane half h of wave w runs chains 2 (w + NW q) + h, q = 0, 1, ..; lane -> node
+            // (a,b) of the degree-DS triangle, stepped by increments as above
+            constexpr int CP = (EB * 2 + NW - 1) / NW;  // chain pairs per wave
+            const int h = lane >> 5, qn = lane & 31;
+            const bool act2 = qn < nfp2(DS);
+            int a2 = 0, b2 = 0;
+            if (act2) {
+                const int ab = (int)dec[np3(DS - 1) + qn];
+                a2 = ab & 255;
+                b2 = ab >> 8;
+            }
+            const int bF2 = __mul24(b2, RS);
+            int idx2 = __mul24(lin2(DS + 1, a2, b2), RS), dst2 = __mul24(lin2(DS, a2, b2), RS);
+            int o22 = idx2 + __mul24(DS + 2 - b2, RS);
+            T wc2 = T(DS + 1 - a2 - b2);
+            const T wa2 = T(a2 + 1), wb2 = T(b2 + 1);
+#pragma unroll
+            for (int s = N - DS; s <= N; ++s) {
+                const int dd = N - s;
+                if (act2 && a2 + b2 <= dd) {
+                    for (int q = 0; q < CP; ++q) {
+                        const int c = 2 * (wave_u + NW * q) + h;
+                        if (c >= ne * 4) break;
+                        const int cbo = __mul24(c, NP * RS);
+                        chain_item(s1 + (cbo + tet(N, s - 1) * RS), s1 + (cbo + tet(N, s) * RS + dst2), idx2, o22,
+                                   wa2, wb2, wc2);
+                    }
+                }
+                wave_lds_sync();
+                idx2 -= bF2;
+                dst2 -= bF2;
+                o22 -= bF2 + RS;
+                wc2 -= T(1);
+            }
         }
         __syncthreads();  // every chain complete before the gather
     } else {
@@ -719,7 +765,7 @@ int stage_launch_wave_md_mixed(int N, int tiling, const StageArgs &a, hipStream_
 constexpr int kStageMaxN = 10;
 // candidate tilings (elements per block, lift organisation), in autotuning order
 constexpr int kStageEB[] = {1, 2, 4};
-constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG, LIFT_GATHER_MFMA, LIFT_WAVE, LIFT_WAVE_GT};
+constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG, LIFT_GATHER_MFMA, LIFT_WAVE, LIFT_WAVE_GT, LIFT_WAVE1};
 
 // tiling code: elements per block + 16 * lift organisation
 constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
@@ -752,10 +798,12 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             using M2 = std::integral_constant<int, LIFT_GATHER_MFMA>;                                  \
             using W3 = std::integral_constant<int, LIFT_WAVE>;                                         \
             using W4 = std::integral_constant<int, LIFT_WAVE_GT>;                                      \
+            using W5 = std::integral_constant<int, LIFT_WAVE1>;                                        \
             one(std::integral_constant<int, 1>{}, W3{});                                               \
             one(std::integral_constant<int, 2>{}, W3{});                                               \
             one(std::integral_constant<int, 1>{}, W4{});                                               \
             one(std::integral_constant<int, 2>{}, W4{});                                               \
+            one(std::integral_constant<int, 1>{}, W5{});                                               \
             one(std::integral_constant<int, 1>{}, M2{});                                               \
             one(std::integral_constant<int, 2>{}, M2{});                                               \
             one(std::integral_constant<int, 4>{}, M2{});                                               \
@@ -795,6 +843,7 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             using P1 = std::integral_constant<int, LIFT_PINGPONG>;                                     \
             using W3 = std::integral_constant<int, LIFT_WAVE>;                                         \
             using W4 = std::integral_constant<int, LIFT_WAVE_GT>;                                      \
+            using W5 = std::integral_constant<int, LIFT_WAVE1>;                                        \
             one(std::integral_constant<int, 1>{}, G0{});                                               \
             one(std::integral_constant<int, 2>{}, G0{});                                               \
             one(std::integral_constant<int, 4>{}, G0{});                                               \
@@ -803,6 +852,7 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             one(std::integral_constant<int, 4>{}, P1{});                                               \
             one(std::integral_constant<int, 1>{}, W3{});                                               \
             one(std::integral_constant<int, 1>{}, W4{});                                               \
+            one(std::integral_constant<int, 1>{}, W5{});                                               \
             return rc;                                                                                 \
         });                                                                                            \
     }                                                                                                  \

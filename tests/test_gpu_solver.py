@@ -91,7 +91,7 @@ def test_autotune_and_tiling_invariance(ctx):
     outs = {}
     import bbdg
 
-    for lift in ("gather", "pingpong", "wave", "wave_gt"):
+    for lift in ("gather", "pingpong", "wave", "wave_gt", "wave1"):
         for eb in (1, 2, 4):
             try:
                 s.set_tuning("f64", s.tiling_code(eb, lift))
@@ -105,11 +105,12 @@ def test_autotune_and_tiling_invariance(ctx):
     # the wave organisations compute every chain item exactly as the gather one
     assert np.array_equal(outs[("wave", 1)], outs[("gather", 1)])
     assert np.array_equal(outs[("wave_gt", 1)], outs[("gather", 1)])
+    assert np.array_equal(outs[("wave1", 1)], outs[("gather", 1)])
     t = s.autotune("f64")
-    assert t in [s.tiling_code(e, lm) for e in (1, 2, 4) for lm in ("gather", "pingpong", "wave", "wave_gt")]
+    assert t in [s.tiling_code(e, lm) for e in (1, 2, 4) for lm in ("gather", "pingpong", "wave", "wave_gt", "wave1")]
     tm = s.autotune("mixed")
     assert tm in [s.tiling_code(e, lm) for e in (1, 2, 4)
-                  for lm in ("gather", "pingpong", "gather_mfma", "wave", "wave_gt")]
+                  for lm in ("gather", "pingpong", "gather_mfma", "wave", "wave_gt", "wave1")]
     assert rel(s.rhs("fused", "f64"), ref) <= TOL64
     for prec in ("f64", "mixed"):
         s.set_tuning(prec, s.tiling_code(1, "pingpong"))
@@ -133,7 +134,7 @@ def test_wave_owned_chains(ctx, pde, N, n):
     ran = 0
     for prec in ("f64", "mixed"):
         for eb in (1, 2):
-            for org in ("wave", "wave_gt"):
+            for org in ("wave", "wave_gt", "wave1"):
                 try:
                     s.set_tuning(prec, s.tiling_code(eb, org))
                 except bbdg.BBError:
