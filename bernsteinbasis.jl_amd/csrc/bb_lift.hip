@@ -295,8 +295,9 @@ template <int N> __device__ constexpr LiftTab<N> kLiftTab = LiftTab<N>();
 
 // Compact field-major layouts with F (1..4, 6) as a template parameter and 7 <= N <= 10: one
 // block of four waves per element, wave f owning face f's chain for all F fields.  The
-// face data become LDS records of F values (field fastest), so that a stencil point is one
-// address and one or two vector accesses for all fields.  The chain is unnormalised (slab
+// face data become LDS records of F values (field fastest; FP64 with an even F in planes of
+// two fields), so that a stencil point is one address and one to three 16-byte accesses for
+// all fields.  The chain is unnormalised (slab
 // 0 = 2 L0 x, no 1/(d+1) per step; lw_chain carries both factors), each lane keeps its node
 // (a,b) of the degree N-1 triangle for the whole chain and steps its offsets by
 // increments, and the steps of a chain need only wave-level ordering.  Every slab of every
@@ -305,8 +306,10 @@ template <int N> __device__ constexpr LiftTab<N> kLiftTab = LiftTab<N>();
 constexpr int kLiftR = 256;
 template <int N, int F, typename TI, typename TC> struct LiftRCfg {
     static constexpr int NP = np3(N), NFP = nfp2(N);
-    static constexpr int RS = rec_stride<TC>(F);  // LDS record stride (values)
-    static constexpr size_t lds = (size_t)4 * (NP + NFP) * RS * sizeof(TC) + (N + 1) * 8;
+    // FP64 with an even field count: planes of two fields (16 bytes per node and plane);
+    // otherwise whole records
+    static constexpr int W = (sizeof(TC) == 8 && F % 2 == 0) ? 2 : F;
+    static constexpr size_t lds = (size_t)4 * (NP + NFP) * F * sizeof(TC) + (N + 1) * 8;
     // below N = 7 the four waves per element leave most lanes idle and the two-wave kernel
     // (k_lift4w) is faster (measured at N = 3, 5: 1.5x)
     static constexpr bool valid = N >= 7 && N <= 10 && lds <= 64 * 1024;
@@ -314,10 +317,11 @@ template <int N, int F, typename TI, typename TC> struct LiftRCfg {
 
 template <int N, int F, typename TI, typename TC, typename TA>
 __global__ __launch_bounds__(kLiftR) void k_lift4r(const TI *__restrict__ x, TI *__restrict__ out, int64_t K) {
-    constexpr int NP = np3(N), NFP = nfp2(N), MI = (NFP + 63) / 64, RS = LiftRCfg<N, F, TI, TC>::RS;
+    constexpr int NP = np3(N), NFP = nfp2(N), MI = (NFP + 63) / 64, W = LiftRCfg<N, F, TI, TC>::W;
+    constexpr int PSS = 4 * NP * W, PSX = 4 * NFP * W;  // plane strides
     const LiftTab<N> &tb = kLiftTab<N>;
-    __shared__ __align__(16) TC S[4 * NP * RS];   // [face][slab node][field]
-    __shared__ __align__(16) TC X[4 * NFP * RS];  // [face][face node][field]
+    __shared__ __align__(16) TC S[4 * NP * F];   // [plane][face][slab node][W fields]
+    __shared__ __align__(16) TC X[4 * NFP * F];  // [plane][face][face node][W fields]
     __shared__ double lw[N + 1];
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid / 64);  // this wave's face
@@ -340,7 +344,7 @@ __global__ __launch_bounds__(kLiftR) void k_lift4r(const TI *__restrict__ x, TI 
             TC v[F];
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = TC(xe[f * 4 * NFP + m]);
-            recv_store<F, RS>(X + __mul24(w * NFP + m, RS), v);
+            plane_store<F, W>(X + __mul24(w * NFP + m, W), PSX, v);
         }
     }
     if (tid <= N) {
@@ -350,36 +354,36 @@ __global__ __launch_bounds__(kLiftR) void k_lift4r(const TI *__restrict__ x, TI 
     }
     wave_lds_sync();
     // L0 (without its 1/2) -> slab 0 of chain w
-    const TC *xw = X + w * NFP * RS;
-    TC *sw = S + w * NP * RS;
+    const TC *xw = X + w * NFP * W;
+    TC *sw = S + w * NP * W;
 #pragma unroll
     for (int mi = 0; mi < MI; ++mi) {
         const int beta = mi * 64 + lane;
         if (beta < NFP) {
-            const L0Rec<TC> pt = l0_prep_rec<N, RS, TC>(l0ab[mi] & 255, l0ab[mi] >> 8);
+            const L0Rec<TC> pt = l0_prep_rec<N, W, TC>(l0ab[mi] & 255, l0ab[mi] >> 8);
             TC v[F], y[F];
-            recv_load<F, RS>(xw + pt.o0, y);
+            plane_load<F, W>(xw + pt.o0, PSX, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = pt.w0 * y[f];
-            recv_load<F, RS>(xw + pt.o1, y);
+            plane_load<F, W>(xw + pt.o1, PSX, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w1, y[f], v[f]);
-            recv_load<F, RS>(xw + pt.o2, y);
+            plane_load<F, W>(xw + pt.o2, PSX, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w2, y[f], v[f]);
-            recv_load<F, RS>(xw + pt.o3, y);
+            plane_load<F, W>(xw + pt.o3, PSX, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w3, y[f], v[f]);
-            recv_load<F, RS>(xw + pt.o4, y);
+            plane_load<F, W>(xw + pt.o4, PSX, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w4, y[f], v[f]);
-            recv_load<F, RS>(xw + pt.o5, y);
+            plane_load<F, W>(xw + pt.o5, PSX, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w5, y[f], v[f]);
-            recv_load<F, RS>(xw + pt.o6, y);
+            plane_load<F, W>(xw + pt.o6, PSX, y);
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = fma(pt.w6, y[f], v[f]);
-            recv_store<F, RS>(sw + __mul24(beta, RS), v);
+            plane_store<F, W>(sw + __mul24(beta, W), PSS, v);
         }
     }
     wave_lds_sync();
@@ -387,27 +391,27 @@ __global__ __launch_bounds__(kLiftR) void k_lift4r(const TI *__restrict__ x, TI 
     // without /(d+1)), lane -> (a,b) fixed, offsets stepped by increments
     {
         const int a = chab & 255, b = chab >> 8;
-        const int bF = __mul24(b, RS);
-        int idxF = __mul24(lin2(N, a, b), RS), dstF = __mul24(lin2(N - 1, a, b), RS);
-        int o2F = idxF + __mul24(N + 1 - b, RS);
+        const int bF = __mul24(b, W);
+        int idxF = __mul24(lin2(N, a, b), W), dstF = __mul24(lin2(N - 1, a, b), W);
+        int o2F = idxF + __mul24(N + 1 - b, W);
         TC wc = TC(N - a - b);
         const TC wa = TC(a + 1), wb = TC(b + 1);
 #pragma unroll
         for (int s = 1; s <= N; ++s) {
             if (lact && a + b <= N - s) {
-                const TC *src = sw + tet(N, s - 1) * RS;
+                const TC *src = sw + tet(N, s - 1) * W;
                 TC x1[F], x2[F], x0[F], v[F];
-                recv_load<F, RS>(src + idxF + RS, x1);
-                recv_load<F, RS>(src + o2F, x2);
-                recv_load<F, RS>(src + idxF, x0);
+                plane_load<F, W>(src + idxF + W, PSS, x1);
+                plane_load<F, W>(src + o2F, PSS, x2);
+                plane_load<F, W>(src + idxF, PSS, x0);
 #pragma unroll
                 for (int f = 0; f < F; ++f) v[f] = fma(wa, x1[f], fma(wb, x2[f], wc * x0[f]));
-                recv_store<F, RS>(sw + tet(N, s) * RS + dstF, v);
+                plane_store<F, W>(sw + tet(N, s) * W + dstF, PSS, v);
             }
             wave_lds_sync();
             idxF -= bF;
             dstF -= bF;
-            o2F -= bF + RS;
+            o2F -= bF + W;
             wc -= TC(1);
         }
     }
@@ -428,7 +432,7 @@ __global__ __launch_bounds__(kLiftR) void k_lift4r(const TI *__restrict__ x, TI 
             const int ff = order[o];
             const TA wt = TA(lw[slab[ff]]);
             TC v[F];
-            recv_load<F, RS>(S + ff * NP * RS + __mul24(go[ff], RS), v);
+            plane_load<F, W>(S + ff * NP * W + __mul24(go[ff], W), PSS, v);
 #pragma unroll
             for (int f = 0; f < F; ++f) acc[f] = fma(wt, TA(v[f]), acc[f]);
         }

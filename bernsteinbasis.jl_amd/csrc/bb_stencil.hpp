@@ -195,6 +195,31 @@ template <int F, int RS = F, typename T> __device__ __forceinline__ void recv_st
     }
 }
 
+// The F values of a node split over F / W planes of W values each, plane c at p + c PS:
+// consecutive nodes are W values apart in every plane (with 16-byte planes, consecutive
+// lanes then meet distinct LDS banks where a whole 32-byte record would meet two-way
+// conflicts).  W = F is the plain record.
+template <int F, int W, typename T> __device__ __forceinline__ void plane_load(const T *p, int PS, T (&v)[F]) {
+    static_assert(F % W == 0, "planes of whole W");
+#pragma unroll
+    for (int c = 0; c < F / W; ++c) {
+        T t[W];
+        recv_load<W, W>(p + c * PS, t);
+#pragma unroll
+        for (int w = 0; w < W; ++w) v[c * W + w] = t[w];
+    }
+}
+template <int F, int W, typename T> __device__ __forceinline__ void plane_store(T *p, int PS, const T (&v)[F]) {
+    static_assert(F % W == 0, "planes of whole W");
+#pragma unroll
+    for (int c = 0; c < F / W; ++c) {
+        T t[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) t[w] = v[c * W + w];
+        recv_store<W, W>(p + c * PS, t);
+    }
+}
+
 // Twice the three reference-coordinate derivatives of G fields of node records F values
 // apart (x + F m holds node m; x points at the group's first field): the 13 stencil
 // points of grad_ref_ld, each read once for the group, the four weighted differences per
