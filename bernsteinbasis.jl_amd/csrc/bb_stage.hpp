@@ -105,7 +105,8 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int NT = ((EB * NP + 63) / 64) * 64;
     static constexpr int RS = rec_stride<T>(F);  // LDS record stride (values)
     // PINGPONG: s1 = state, then chain A; s2 = flux, then chain B
-    // GATHER:   s1 = state, then all chain vectors (EB F 4 Np); s2 = flux
+    // GATHER:   s1 = state, then all chain vectors (EB 4 Np records); s2 = flux
+    // WAVE*:    s0 = state, s1 = all chain vectors, s2 = flux
     static constexpr bool WAVEF = LM == LIFT_WAVE || LM == LIFT_WAVE_GT || LM == LIFT_WAVE1;
     static constexpr bool GATHER = LM == LIFT_GATHER || LM == LIFT_GATHER_MFMA || WAVEF;
     static constexpr int R0 = WAVEF ? EB * RS * NP : 0;  // separate state buffer (WAVE)
@@ -116,7 +117,7 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int TAB = TABG + 4 * NP;                      // then the gather offsets
     static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
     static constexpr int R3 = MD ? EB * 4 * RS * kTraceMax : 0;  // neighbour traces of other degrees
-    // index tables in LDS, or read from global memory (WAVE_GT)
+    // index tables in LDS, or read from global memory (WAVE_GT, WAVE1)
     static constexpr bool TABL = LM != LIFT_WAVE_GT && LM != LIFT_WAVE1;
     static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (TABL ? (size_t)TAB16 * 16 : 0) + (N + 1) * 8;
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
