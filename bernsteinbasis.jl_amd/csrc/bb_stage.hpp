@@ -119,6 +119,8 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int R3 = MD ? EB * 4 * RS * kTraceMax : 0;  // neighbour traces of other degrees
     // index tables in LDS, or read from global memory (WAVE_GT, WAVE1)
     static constexpr bool TABL = LM != LIFT_WAVE_GT && LM != LIFT_WAVE1;
+    // prologue loads issued in the order they are consumed (FP64 single-degree kernels)
+    static constexpr bool ORDERED = !MIXED && !MD;
     static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (TABL ? (size_t)TAB16 * 16 : 0) + (N + 1) * 8;
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
     static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
@@ -245,9 +247,10 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     const int64_t eg = EB == 1 ? e0 : e0 + el;
 
     // ---- prologue: every global read the stage needs is issued here, before the first
-    // LDS work, so that its latency overlaps the volume phase:
-    //   index tables -> LDS (one 16-byte word per thread), the element state and the RK
-    //   residual of the owned node, and the neighbour traces of this thread's face nodes
+    // LDS work, so that its latency overlaps the volume phase: index tables (into LDS, or
+    // the per-lane entries of the later phases from global memory), the element state and
+    // the RK residual of the owned node, and the neighbour traces of this thread's face
+    // nodes.
     int pk = 0;   // this node's (i, j, k)
     uint2 gp{};   // where the four faces' chains land on this node
     if constexpr (Cfg::TABL) {
@@ -273,21 +276,6 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             for (int kk = 0; kk < KT; ++kk) amf[kk] = ap[kk];
         }
     }
-    using TR = T;
-    TR *res = reinterpret_cast<TR *>(A.res);
-    double qv[F];
-    TR rv[F];
-    double invJ = 0.0, dte = 0.0;  // read here: a load issued in the epilogue stalls the workgroup's end
-    if (own) {
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-            qv[f] = A.q_in[(eg * F + f) * NP + n];
-            rv[f] = (A.first || A.rhs_only) ? TR(0) : res[(eg * F + f) * NP + n];
-        }
-        invJ = A.invJ[eg];
-        dte = MD ? A.dt * A.dte[eg] : A.dt;
-    }
-    __syncthreads();  // tables visible
     // face-node work items of this thread (at most RF of them).  WAVE: chain c = (element,
     // face) belongs to wave c mod NW, its nodes to the lanes; otherwise items are dealt to
     // the threads in order.
@@ -312,32 +300,120 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     // read here with the other global reads -- own face node of each flux item, the face
     // node (a,b) of each L0 item and the chain lane's node of the degree N-1 triangle
     int fvp[RF], l0p[MI], chp = 0;
-    if constexpr (!Cfg::TABL) {
+    using TR = T;
+    TR *res = reinterpret_cast<TR *>(A.res);
+    double qv[F];
+    TR rv[F];
+    double invJ = 0.0, dte = 0.0;  // read here: a load issued in the epilogue stalls the workgroup's end
+    int2 cnv[RF];
+    T nb[RF][F];
+    if constexpr (Cfg::ORDERED) {
+        // Vector loads complete in issue order (one counter), so a wait for one load is a
+        // wait for every load issued before it.  Here they are issued in the order they
+        // are consumed -- tables (with the neighbour node index of each face item), state,
+        // neighbour traces, residual -- without branches around them (a branch makes the
+        // merged wait counts cover later loads too): the state barrier then waits for the
+        // state alone and the traces are waited for in the flux phase.  This pays for the
+        // FP64 single-degree kernels, whose occupancy LDS limits; elsewhere the longer-lived
+        // registers cost more occupancy than the overlap gains.
+        //
+        // Items whose neighbour is an interior element (nbi) read its trace; the others read
+        // their own element's node 0 and ignore it.
+        int nvr[RF];
+        bool nbi[RF];
+        const double *nbase[RF];
+        auto nb_index = [&](int r) {
+            int e2, phi, m;
+            const bool ok = fitem(r, e2, phi, m);
+            e2 = e2 < ne ? e2 : 0;  // phi is always 0..3; with wave-owned chains e2 stays uniform
+            cnv[r] = A.conn[(e0 + e2) * 4 + phi];
+            nbi[r] = ok && ((cnv[r].y >> 5) & 7) == 0;
+            nvr[r] = nvid[nbi[r] ? ((cnv[r].y & 3) * 6 + ((cnv[r].y >> 2) & 7)) * NFP + m : 0];
+            nbase[r] = A.q_in + (nbi[r] ? (int64_t)cnv[r].x : eg) * F * NP;
+        };
+        if constexpr (!Cfg::TABL) {
+#pragma unroll
+            for (int r = 0; r < RF; ++r) nb_index(r);
+#pragma unroll
+            for (int r = 0; r < RF; ++r) {
+                int e2, phi, m;
+                fvp[r] = fvid[fitem(r, e2, phi, m) ? phi * NFP + m : 0];
+            }
+            // (entries of lanes without an item are read at index 0 and never used: no branch
+            // around the loads, whose merged wait counts would otherwise cover later loads)
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi) {
+                const int beta = mi * 64 + (tid & 63);
+                l0p[mi] = dec[np3(N - 1) + (beta < NFP ? beta : 0)];
+            }
+            chp = dec[np3(N - 2) + ((tid & 63) < nfp2(N - 1) ? (tid & 63) : 0)];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // every thread loads (threads past the block's nodes read node 0 of element e0): no
+        // branch around the loads
+#pragma unroll
+        for (int f = 0; f < F; ++f) qv[f] = A.q_in[(eg * F + f) * NP + n];
+        invJ = A.invJ[eg];
+        dte = A.dt;
+        if constexpr (Cfg::TABL) {
+            __syncthreads();  // tables visible
+#pragma unroll
+            for (int r = 0; r < RF; ++r) nb_index(r);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < RF; ++r) {
+            const double *qn = nbase[r] + (nbi[r] ? nvr[r] : 0);
+#pragma unroll
+            for (int f = 0; f < F; ++f) nb[r][f] = qn[f * NP];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // the residual is loaded also where it is not used (first stage, RHS only) and then
+        // replaced, again so that no branch surrounds the loads
+        {
+            const bool zero = A.first || A.rhs_only;
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                const TR x = res[(eg * F + f) * NP + n];
+                rv[f] = zero ? TR(0) : x;
+            }
+        }
+    } else {
+        if (own) {
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                qv[f] = A.q_in[(eg * F + f) * NP + n];
+                rv[f] = (A.first || A.rhs_only) ? TR(0) : res[(eg * F + f) * NP + n];
+            }
+            invJ = A.invJ[eg];
+            dte = MD ? A.dt * A.dte[eg] : A.dt;
+        }
+        __syncthreads();  // tables visible
+        if constexpr (!Cfg::TABL) {
+#pragma unroll
+            for (int r = 0; r < RF; ++r) {
+                int e2, phi, m;
+                fvp[r] = fitem(r, e2, phi, m) ? fvid[phi * NFP + m] : 0;
+            }
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi) {
+                const int beta = mi * 64 + (tid & 63);
+                l0p[mi] = beta < NFP ? dec[np3(N - 1) + beta] : 0;
+            }
+            if ((tid & 63) < nfp2(N - 1)) chp = dec[np3(N - 2) + (tid & 63)];
+        }
 #pragma unroll
         for (int r = 0; r < RF; ++r) {
             int e2, phi, m;
-            fvp[r] = fitem(r, e2, phi, m) ? fvid[phi * NFP + m] : 0;
-        }
+            if (fitem(r, e2, phi, m)) {
+                cnv[r] = A.conn[(e0 + e2) * 4 + phi];
+                if (((cnv[r].y >> 5) & 7) == 0 && (!MD || ((cnv[r].y >> 8) & 15) == N)) {
+                    const int nvi = nvid[((cnv[r].y & 3) * 6 + ((cnv[r].y >> 2) & 7)) * NFP + m];
+                    const double *qn = MD ? A.q_nb + A.nboff[(e0 + e2) * 4 + phi] + nvi
+                                          : A.q_in + (int64_t)cnv[r].x * F * NP + nvi;
 #pragma unroll
-        for (int mi = 0; mi < MI; ++mi) {
-            const int beta = mi * 64 + (tid & 63);
-            l0p[mi] = beta < NFP ? dec[np3(N - 1) + beta] : 0;
-        }
-        if ((tid & 63) < nfp2(N - 1)) chp = dec[np3(N - 2) + (tid & 63)];
-    }
-    T nb[RF][F];
-    int2 cnv[RF];
-#pragma unroll
-    for (int r = 0; r < RF; ++r) {
-        int e2, phi, m;
-        if (fitem(r, e2, phi, m)) {
-            cnv[r] = A.conn[(e0 + e2) * 4 + phi];
-            if (((cnv[r].y >> 5) & 7) == 0 && (!MD || ((cnv[r].y >> 8) & 15) == N)) {
-                const int nvi = nvid[((cnv[r].y & 3) * 6 + ((cnv[r].y >> 2) & 7)) * NFP + m];
-                const double *qn = MD ? A.q_nb + A.nboff[(e0 + e2) * 4 + phi] + nvi
-                                      : A.q_in + (int64_t)cnv[r].x * F * NP + nvi;
-#pragma unroll
-                for (int f = 0; f < F; ++f) nb[r][f] = T(qn[f * NP]);
+                    for (int f = 0; f < F; ++f) nb[r][f] = T(qn[f * NP]);
+                }
             }
         }
     }
