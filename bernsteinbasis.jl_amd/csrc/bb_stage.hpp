@@ -121,6 +121,10 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr bool TABL = LM != LIFT_WAVE_GT && LM != LIFT_WAVE1;
     // prologue loads issued in the order they are consumed (FP64 single-degree kernels)
     static constexpr bool ORDERED = !MIXED && !MD;
+    // uniform per-element data read in the prologue (single-degree kernels): geometric
+    // factors with one element per workgroup, face scalings with wave-owned chains
+    static constexpr bool PRE_GEO = EB == 1 && !MD;
+    static constexpr bool PRE_NJ = WAVEF && !MD;
     static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (TABL ? (size_t)TAB16 * 16 : 0) + (N + 1) * 8;
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
     static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
@@ -435,6 +439,46 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             }
         }
     }
+    // per-element data with a uniform index, as scalar loads issued here rather than at the
+    // start of the phase that uses them (each element's data is read once, so the first
+    // read always goes to memory): the halved geometric factors with one element per
+    // workgroup, the face scalings nJ of each item with wave-owned chains.  Scalar loads
+    // return out of order, so any later wait for one waits for all: issued after the other
+    // prologue loads, they complete under the state loads that the state barrier awaits.
+    // (The mixed-degree kernels keep these loads in the phases that use them: there the
+    // pinning below turned their later uniform table reads into vector loads, and the extra
+    // registers cost a wave per SIMD.)
+    T gh[9];
+    if constexpr (Cfg::PRE_GEO) {
+        const T *g = reinterpret_cast<const T *>(A.geoh) + e0 * 9;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) gh[c] = g[c];
+    }
+    double njw[Cfg::PRE_NJ ? RF : 1][5];
+    if constexpr (Cfg::PRE_NJ) {
+#pragma unroll
+        for (int r = 0; r < RF; ++r) {
+            // chain c = (element, face) of item r is uniform across the wave
+            const int c = __builtin_amdgcn_readfirstlane(wave_u + NW * (r / MI));
+            const int e2 = (c >> 2) < ne ? (c >> 2) : 0;
+            const double *nj = A.nJ + ((e0 + e2) * 4 + (c & 3)) * 5;
+#pragma unroll
+            for (int c5 = 0; c5 < 5; ++c5) njw[r][c5] = nj[c5];
+        }
+    }
+    // the loaded values are taken here (in scalar registers), so that the loads are not
+    // moved into the later phases that use them
+    auto pin_scalar = [](auto &x) { asm volatile("" ::"s"(x)); };
+    if constexpr (Cfg::PRE_GEO) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) pin_scalar(gh[c]);
+    }
+    if constexpr (Cfg::PRE_NJ) {
+#pragma unroll
+        for (int r = 0; r < RF; ++r)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) pin_scalar(njw[r][c]);
+    }
     if (own) {
         T v[F];
 #pragma unroll
@@ -455,11 +499,13 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     if (own) {
         // halved geometric factors (grad2_rec returns twice the reference derivatives),
         // prepared on the host in T: with one element per workgroup they are scalar loads
-        // that the FMAs read from scalar registers
-        const T *g = reinterpret_cast<const T *>(A.geoh) + eg * 9;
-        T gh[9];
+        // (issued in the prologue by single-degree kernels) that the FMAs read from scalar
+        // registers
+        if constexpr (!Cfg::PRE_GEO) {
+            const T *g = reinterpret_cast<const T *>(A.geoh) + eg * 9;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) gh[c] = g[c];
+            for (int c = 0; c < 9; ++c) gh[c] = g[c];
+        }
         // fields in groups of G (one 16- or 8-byte word per stencil point and group)
         constexpr int G = (sizeof(T) == 4 && F % 4 == 0) ? 4 : 2;
         const T *xs = sst + __mul24(el, NP * RS);
@@ -532,7 +578,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 #pragma unroll
                 for (int f = 0; f < F; ++f) d[f] = T(A.sign[f]) * um[f] - um[f];
             }
-            const double *nj = A.nJ + ((e0 + e2) * 4 + phi) * 5;
+            double nj[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) nj[c] = Cfg::PRE_NJ ? njw[Cfg::PRE_NJ ? r : 0][c] : A.nJ[((e0 + e2) * 4 + phi) * 5 + c];
             P::template flux<T>(A.flux_kind, d, T(nj[0]), T(nj[1]), T(nj[2]), T(nj[3]), T(nj[4]), o);
             recv_store<F, RS>(s2 + __mul24((e2 * 4 + phi) * NFP + m, RS), o);
         }
