@@ -311,6 +311,34 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     double invJ = 0.0, dte = 0.0;  // read here: a load issued in the epilogue stalls the workgroup's end
     int2 cnv[RF];
     T nb[RF][F];
+    // per-element data with a uniform index, as scalar loads issued in the prologue rather
+    // than at the start of the phase that uses them (each element's data is read once, so
+    // the first read always goes to memory): the halved geometric factors with one element
+    // per workgroup, the face scalings nJ of each item with wave-owned chains.  Scalar loads
+    // return out of order, so any later wait for one waits for all; where they are issued
+    // depends on the prologue order below.  (The mixed-degree kernels keep these loads in
+    // the phases that use them: there the pinning below turned their later uniform table
+    // reads into vector loads, and the extra registers cost a wave per SIMD.)
+    T gh[9];
+    double njw[Cfg::PRE_NJ ? RF : 1][5];
+    auto uniform_loads = [&]() {
+        if constexpr (Cfg::PRE_GEO) {
+            const T *g = reinterpret_cast<const T *>(A.geoh) + e0 * 9;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) gh[c] = g[c];
+        }
+        if constexpr (Cfg::PRE_NJ) {
+#pragma unroll
+            for (int r = 0; r < RF; ++r) {
+                // chain c = (element, face) of item r is uniform across the wave
+                const int c = __builtin_amdgcn_readfirstlane(wave_u + NW * (r / MI));
+                const int e2 = (c >> 2) < ne ? (c >> 2) : 0;
+                const double *nj = A.nJ + ((e0 + e2) * 4 + (c & 3)) * 5;
+#pragma unroll
+                for (int c5 = 0; c5 < 5; ++c5) njw[r][c5] = nj[c5];
+            }
+        }
+    };
     if constexpr (Cfg::ORDERED) {
         // Vector loads complete in issue order (one counter), so a wait for one load is a
         // wait for every load issued before it.  Here they are issued in the order they
@@ -382,6 +410,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 rv[f] = zero ? TR(0) : x;
             }
         }
+        // last: an earlier wait for a scalar load (the connectivity above) would wait for
+        // these too, and the state barrier waits for the state loads anyway
+        uniform_loads();
     } else {
         if (own) {
 #pragma unroll
@@ -392,7 +423,22 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             invJ = A.invJ[eg];
             dte = MD ? A.dt * A.dte[eg] : A.dt;
         }
-        __syncthreads();  // tables visible
+        // single-degree kernels: the connectivity of each item (a scalar load with wave-owned
+        // chains) and the uniform per-element data are issued with the state loads, so that
+        // their latencies overlap rather than follow one another; the barrier is needed only
+        // for index tables in LDS
+        constexpr bool EARLY = !MD;
+        if constexpr (EARLY) {
+#pragma unroll
+            for (int r = 0; r < RF; ++r) {
+                int e2, phi, m;
+                fitem(r, e2, phi, m);
+                e2 = e2 < ne ? e2 : 0;
+                cnv[r] = A.conn[(e0 + e2) * 4 + phi];
+            }
+            uniform_loads();
+        }
+        if constexpr (Cfg::TABL || !EARLY) __syncthreads();  // tables visible
         if constexpr (!Cfg::TABL) {
 #pragma unroll
             for (int r = 0; r < RF; ++r) {
@@ -410,7 +456,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         for (int r = 0; r < RF; ++r) {
             int e2, phi, m;
             if (fitem(r, e2, phi, m)) {
-                cnv[r] = A.conn[(e0 + e2) * 4 + phi];
+                if constexpr (!EARLY) cnv[r] = A.conn[(e0 + e2) * 4 + phi];
                 if (((cnv[r].y >> 5) & 7) == 0 && (!MD || ((cnv[r].y >> 8) & 15) == N)) {
                     const int nvi = nvid[((cnv[r].y & 3) * 6 + ((cnv[r].y >> 2) & 7)) * NFP + m];
                     const double *qn = MD ? A.q_nb + A.nboff[(e0 + e2) * 4 + phi] + nvi
@@ -437,33 +483,6 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 for (int f = 0; f < F; ++f) v[f] = T(qn[f * npn]);
                 recv_store<F, RS>(s3 + __mul24((e2 * 4 + phi) * kTraceMax + jn, RS), v);
             }
-        }
-    }
-    // per-element data with a uniform index, as scalar loads issued here rather than at the
-    // start of the phase that uses them (each element's data is read once, so the first
-    // read always goes to memory): the halved geometric factors with one element per
-    // workgroup, the face scalings nJ of each item with wave-owned chains.  Scalar loads
-    // return out of order, so any later wait for one waits for all: issued after the other
-    // prologue loads, they complete under the state loads that the state barrier awaits.
-    // (The mixed-degree kernels keep these loads in the phases that use them: there the
-    // pinning below turned their later uniform table reads into vector loads, and the extra
-    // registers cost a wave per SIMD.)
-    T gh[9];
-    if constexpr (Cfg::PRE_GEO) {
-        const T *g = reinterpret_cast<const T *>(A.geoh) + e0 * 9;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) gh[c] = g[c];
-    }
-    double njw[Cfg::PRE_NJ ? RF : 1][5];
-    if constexpr (Cfg::PRE_NJ) {
-#pragma unroll
-        for (int r = 0; r < RF; ++r) {
-            // chain c = (element, face) of item r is uniform across the wave
-            const int c = __builtin_amdgcn_readfirstlane(wave_u + NW * (r / MI));
-            const int e2 = (c >> 2) < ne ? (c >> 2) : 0;
-            const double *nj = A.nJ + ((e0 + e2) * 4 + (c & 3)) * 5;
-#pragma unroll
-            for (int c5 = 0; c5 < 5; ++c5) njw[r][c5] = nj[c5];
         }
     }
     // the loaded values are taken here (in scalar registers), so that the loads are not
