@@ -310,3 +310,37 @@ def test_errors_are_reported(ctx):
     lay = bbdg._lib.bb_layout(20, 1, 0)
     rc = bbdg.lib().bb_deriv_apply(ctx.h, 0, 21, 1, 2, 0, x.ptr, lay, x.ptr, lay)
     assert rc == bbdg._lib.BB_ERR_ARG and b"outside" in bbdg.lib().bb_last_error(ctx.h)
+
+
+def test_empty_batch_is_a_no_op(ctx):
+    """K = 0 columns (an empty batch): every operator entry point returns BB_OK and writes
+    nothing (the output keeps its sentinel values)"""
+    import bbdg
+    from bbdg import _lib as B
+
+    lib = bbdg.lib()
+    N, F = 4, 2
+    n, nf = np3(N), nfp(N)
+    x = bbdg.DeviceArray.from_numpy(np.ones(4 * F * max(n, 4 * nf)))
+    geo = bbdg.DeviceArray.from_numpy(np.ones(9))
+    sentinel = np.full(F * n, 7.0)
+    outs = [bbdg.DeviceArray.from_numpy(sentinel.copy()) for _ in range(3)]
+    vol = B.bb_layout(F * n, 1, n)
+    face4 = B.bb_layout(F * 4 * nf, 1, 4 * nf)
+    face1 = B.bb_layout(F * nf, 1, nf)
+    for dt in (B.BB_F64, B.BB_F32):
+        calls = {
+            "deriv": lambda: lib.bb_deriv_apply(ctx.h, 0, N, F, 0, dt, x.ptr, vol, outs[0].ptr, vol),
+            "grad": lambda: lib.bb_grad_apply(ctx.h, N, F, 0, dt, x.ptr, vol, geo.ptr, outs[0].ptr, outs[1].ptr,
+                                              outs[2].ptr, vol),
+            "lift": lambda: lib.bb_lift_apply(ctx.h, N, F, 0, dt, 0, x.ptr, face4, outs[0].ptr, vol),
+            "face_lift": lambda: lib.bb_face_lift_apply(ctx.h, N, F, 0, dt, dt, 0, x.ptr, face1, outs[0].ptr, vol),
+            "elevate": lambda: lib.bb_elevate_apply(ctx.h, N, F, 0, dt, x.ptr, face1, outs[0].ptr, face1),
+            "gather": lambda: lib.bb_face_gather(ctx.h, N, F, 0, dt, x.ptr, vol, outs[0].ptr, face4),
+        }
+        for name, call in calls.items():
+            rc = call()
+            assert rc == B.BB_OK, (name, dt, rc, lib.bb_last_error(ctx.h))
+    ctx.sync()
+    for o in outs:
+        assert np.array_equal(o.numpy(), sentinel)
