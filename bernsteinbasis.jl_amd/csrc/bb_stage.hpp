@@ -121,10 +121,11 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr bool TABL = LM != LIFT_WAVE_GT && LM != LIFT_WAVE1;
     // prologue loads issued in the order they are consumed (FP64 single-degree kernels)
     static constexpr bool ORDERED = !MIXED && !MD;
-    // uniform per-element data read in the prologue (single-degree kernels): geometric
-    // factors with one element per workgroup, face scalings with wave-owned chains
-    static constexpr bool PRE_GEO = EB == 1 && !MD;
-    static constexpr bool PRE_NJ = WAVEF && !MD;
+    // uniform per-element data read in the prologue: geometric factors with one element
+    // per workgroup, face scalings with wave-owned chains (mixed-degree kernels: with
+    // wave-owned chains only, and without the pinning below)
+    static constexpr bool PRE_GEO = EB == 1 && (!MD || WAVEF);
+    static constexpr bool PRE_NJ = WAVEF;
     static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (TABL ? (size_t)TAB16 * 16 : 0) + (N + 1) * 8;
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
     static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
@@ -316,8 +317,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     // the first read always goes to memory): the halved geometric factors with one element
     // per workgroup, the face scalings nJ of each item with wave-owned chains.  Scalar loads
     // return out of order, so any later wait for one waits for all; where they are issued
-    // depends on the prologue order below.  (The mixed-degree kernels keep these loads in
-    // the phases that use them: there the pinning below turned their later uniform table
+    // depends on the prologue order below.  (Mixed-degree kernels read them here only with
+    // wave-owned chains, and do not pin them: there the pinning turned later uniform table
     // reads into vector loads, and the extra registers cost a wave per SIMD.)
     T gh[9];
     double njw[Cfg::PRE_NJ ? RF : 1][5];
@@ -423,11 +424,12 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             invJ = A.invJ[eg];
             dte = MD ? A.dt * A.dte[eg] : A.dt;
         }
-        // single-degree kernels: the connectivity of each item (a scalar load with wave-owned
-        // chains) and the uniform per-element data are issued with the state loads, so that
-        // their latencies overlap rather than follow one another; the barrier is needed only
-        // for index tables in LDS
-        constexpr bool EARLY = !MD;
+        // the connectivity of each item (a scalar load with wave-owned chains) and the uniform
+        // per-element data are issued with the state loads, so that their latencies overlap
+        // rather than follow one another (mixed-degree kernels: with wave-owned chains only;
+        // their other organisations hold per-thread connectivity in registers); the barrier
+        // is needed only for index tables in LDS
+        constexpr bool EARLY = !MD || Cfg::WAVEF;
         if constexpr (EARLY) {
 #pragma unroll
             for (int r = 0; r < RF; ++r) {
@@ -488,11 +490,11 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     // the loaded values are taken here (in scalar registers), so that the loads are not
     // moved into the later phases that use them
     auto pin_scalar = [](auto &x) { asm volatile("" ::"s"(x)); };
-    if constexpr (Cfg::PRE_GEO) {
+    if constexpr (Cfg::PRE_GEO && !MD) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) pin_scalar(gh[c]);
     }
-    if constexpr (Cfg::PRE_NJ) {
+    if constexpr (Cfg::PRE_NJ && !MD) {
 #pragma unroll
         for (int r = 0; r < RF; ++r)
 #pragma unroll
