@@ -106,13 +106,18 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int RS = rec_stride<T>(F);  // LDS record stride (values)
     // PINGPONG: s1 = state, then chain A; s2 = flux, then chain B
     // GATHER:   s1 = state, then all chain vectors (EB 4 Np records); s2 = flux
-    // WAVE*:    s0 = state, s1 = all chain vectors, s2 = flux
+    // WAVE*:    s0 = state, s1 = all chain vectors, s2 = flux -- or, with FXA, the flux of
+    //           each chain inside that chain's own region of s1 (records NFP .. 2 NFP - 1,
+    //           i.e. slab 1 onwards): the wave that owns the chain writes its flux, reads it
+    //           in L0 (which writes slab 0) and only then starts the chain, so no other
+    //           buffer is needed
     static constexpr bool WAVEF = LM == LIFT_WAVE || LM == LIFT_WAVE_GT || LM == LIFT_WAVE1;
     static constexpr bool GATHER = LM == LIFT_GATHER || LM == LIFT_GATHER_MFMA || WAVEF;
     static constexpr int R0 = WAVEF ? EB * RS * NP : 0;  // separate state buffer (WAVE)
     static constexpr int R1 = GATHER ? EB * RS * 4 * NP
                               : ((EB * RS * NP > EB * RS * 4 * NFP) ? EB * RS * NP : EB * RS * 4 * NFP);
-    static constexpr int R2 = EB * RS * 4 * NFP;
+    static constexpr bool FXA = WAVEF && 2 * NFP <= NP;
+    static constexpr int R2 = FXA ? 0 : EB * RS * 4 * NFP;
     static constexpr int TABG = (2 * NP + 28 * NFP + 3) / 4 * 4;  // ijk, dec, own / neighbour face nodes
     static constexpr int TAB = TABG + 4 * NP;                      // then the gather offsets
     static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
@@ -227,7 +232,10 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     constexpr int F = Cfg::F, NP = Cfg::NP, NFP = Cfg::NFP, NT = Cfg::NT, RS = Cfg::RS;
     // every LDS array holds node records of F values (field fastest): see recv_load
     __shared__ __align__(16) T s1[Cfg::R1];
-    __shared__ __align__(16) T s2[Cfg::R2];
+    __shared__ __align__(16) T s2[Cfg::R2 ? Cfg::R2 : 1];
+    // flux record of chain c (= 4 element + face), face node m: fxb + (c FXS + m) RS
+    T *const fxb = Cfg::FXA ? s1 + NFP * RS : s2;
+    constexpr int FXS = Cfg::FXA ? NP : NFP;
     __shared__ __align__(16) T s0[Cfg::WAVEF ? Cfg::R0 : 1];
     T *sst = Cfg::WAVEF ? s0 : s1;  // the element state in LDS: [element][node][field]
     __shared__ __align__(16) T s3[MD ? Cfg::R3 : 1];
@@ -603,7 +611,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 #pragma unroll
             for (int c = 0; c < 5; ++c) nj[c] = Cfg::PRE_NJ ? njw[Cfg::PRE_NJ ? r : 0][c] : A.nJ[((e0 + e2) * 4 + phi) * 5 + c];
             P::template flux<T>(A.flux_kind, d, T(nj[0]), T(nj[1]), T(nj[2]), T(nj[3]), T(nj[4]), o);
-            recv_store<F, RS>(s2 + __mul24((e2 * 4 + phi) * NFP + m, RS), o);
+            recv_store<F, RS>(fxb + __mul24((e2 * 4 + phi) * FXS + m, RS), o);
         }
     }
     if constexpr (Cfg::WAVEF) wave_lds_sync();
@@ -678,7 +686,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     } else {
         auto l0_item = [&](int c, int beta, int ab) {
             const L0Rec<T> pt = l0_prep_rec<N, RS, T>(ab & 255, ab >> 8);
-            const T *x = s2 + __mul24(c, NFP * RS);
+            const T *x = fxb + __mul24(c, FXS * RS);
             T v[F], y[F];
             // the 7-point sum in the order of l0_eval, as explicit fma
             recv_load<F, RS>(x + pt.o0, y);
