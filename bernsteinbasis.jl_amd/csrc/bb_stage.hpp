@@ -11,9 +11,11 @@
 //   one is written), res [K][F][Np] (FP64, or FP32 in mixed precision), geo / 2 [K][9],
 //   1/J [K], nJ [K][4][5], conn [K][4] = {neighbour, face | perm << 2 | bc << 5}.
 //
-// LDS (T = FP64, or FP32 in mixed precision), every array as node records of F values:
-//   s1: the element's coefficients, later chain buffer A
-//   s2: interface flux (4 faces x Nfp records), later chain buffer B
+// LDS (T = FP64, or FP32 in mixed precision), every array as node records of F values;
+// the use of each buffer depends on the lift organisation (StageCfg below): the element's
+// coefficients, the interface flux (4 faces x Nfp records), and the chain vectors (two
+// buffers, or every slab of every chain; with wave-owned chains each chain's flux sits in
+// its own chain region until the chain overwrites it)
 // Each thread owns one volume node: its F volume terms and F lift accumulators stay in
 // registers (accumulators always FP64 -- the signed binomially weighted sum of the four
 // faces' chains, where cancellation happens), so no LDS scatter and no atomics.
