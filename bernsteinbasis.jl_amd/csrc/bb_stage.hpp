@@ -518,46 +518,9 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     }
     __syncthreads();  // state visible
 
-    // ---- volume term: 3 reference derivatives per field, geometric combine, PDE flux Jacobian
-    T vol[F];
-#pragma unroll
-    for (int f = 0; f < F; ++f) vol[f] = T(0);
-    if constexpr (Cfg::TABL) {
-        pk = ijk[n];
-        if constexpr (Cfg::GATHER) gp = reinterpret_cast<const uint2 *>(tab + Cfg::TABG)[n];
-    }
-    const int i = pk & 15, j = (pk >> 4) & 15, k = pk >> 8;
-    if (own) {
-        // halved geometric factors (grad2_rec returns twice the reference derivatives),
-        // prepared on the host in T: with one element per workgroup they are scalar loads
-        // (issued in the prologue by single-degree kernels) that the FMAs read from scalar
-        // registers
-        if constexpr (!Cfg::PRE_GEO) {
-            const T *g = reinterpret_cast<const T *>(A.geoh) + eg * 9;
-#pragma unroll
-            for (int c = 0; c < 9; ++c) gh[c] = g[c];
-        }
-        // fields in groups of G (one 16- or 8-byte word per stencil point and group)
-        constexpr int G = (sizeof(T) == 4 && F % 4 == 0) ? 4 : 2;
-        const T *xs = sst + __mul24(el, NP * RS);
-#pragma unroll
-        for (int g0 = 0; g0 < F; g0 += G) {
-            T dr[G], ds[G], dt[G];
-            grad2_rec<N, RS, G, T>(xs + g0, n, i, j, k, dr, ds, dt);
-#pragma unroll
-            for (int q = 0; q < G; ++q) {
-                const T gx = fma(gh[2], dt[q], fma(gh[1], ds[q], gh[0] * dr[q]));
-                const T gy = fma(gh[5], dt[q], fma(gh[4], ds[q], gh[3] * dr[q]));
-                const T gz = fma(gh[8], dt[q], fma(gh[7], ds[q], gh[6] * dr[q]));
-                P::template vol_add<T>(g0 + q, gx, gy, gz, vol);
-            }
-            // one group at a time: hoisting the next group's reads above this group's
-            // arithmetic would hold all 13 F values per node in registers at once
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-
-    // ---- interface flux at every face node of the block's elements -> s2
+    // ---- interface flux at every face node of the block's elements -> s2 (or the chain
+    // buffer).  It runs before the volume term: the neighbour values it consumes then die
+    // before the volume phase's registers are live (mixed Maxwell N=9: 87 -> 81 VGPRs)
 #pragma unroll
     for (int r = 0; r < RF; ++r) {
         int e2, phi, m;
@@ -616,6 +579,45 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             recv_store<F, RS>(fxb + __mul24((e2 * 4 + phi) * FXS + m, RS), o);
         }
     }
+    // ---- volume term: 3 reference derivatives per field, geometric combine, PDE flux Jacobian
+    T vol[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) vol[f] = T(0);
+    if constexpr (Cfg::TABL) {
+        pk = ijk[n];
+        if constexpr (Cfg::GATHER) gp = reinterpret_cast<const uint2 *>(tab + Cfg::TABG)[n];
+    }
+    const int i = pk & 15, j = (pk >> 4) & 15, k = pk >> 8;
+    if (own) {
+        // halved geometric factors (grad2_rec returns twice the reference derivatives),
+        // prepared on the host in T: with one element per workgroup they are scalar loads
+        // (issued in the prologue by single-degree kernels) that the FMAs read from scalar
+        // registers
+        if constexpr (!Cfg::PRE_GEO) {
+            const T *g = reinterpret_cast<const T *>(A.geoh) + eg * 9;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) gh[c] = g[c];
+        }
+        // fields in groups of G (one 16- or 8-byte word per stencil point and group)
+        constexpr int G = (sizeof(T) == 4 && F % 4 == 0) ? 4 : 2;
+        const T *xs = sst + __mul24(el, NP * RS);
+#pragma unroll
+        for (int g0 = 0; g0 < F; g0 += G) {
+            T dr[G], ds[G], dt[G];
+            grad2_rec<N, RS, G, T>(xs + g0, n, i, j, k, dr, ds, dt);
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                const T gx = fma(gh[2], dt[q], fma(gh[1], ds[q], gh[0] * dr[q]));
+                const T gy = fma(gh[5], dt[q], fma(gh[4], ds[q], gh[3] * dr[q]));
+                const T gz = fma(gh[8], dt[q], fma(gh[7], ds[q], gh[6] * dr[q]));
+                P::template vol_add<T>(g0 + q, gx, gy, gz, vol);
+            }
+            // one group at a time: hoisting the next group's reads above this group's
+            // arithmetic would hold all 13 F values per node in registers at once
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
     if constexpr (Cfg::WAVEF) wave_lds_sync();
     else __syncthreads();
 
