@@ -598,8 +598,12 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 #pragma unroll
             for (int c = 0; c < 9; ++c) gh[c] = g[c];
         }
-        // fields in groups of G (one 16- or 8-byte word per stencil point and group)
-        constexpr int G = (sizeof(T) == 4 && F % 4 == 0) ? 4 : 2;
+        // fields in groups of G = 2 (one 8- or 16-byte word per stencil point and group).
+        // The 13 stencil points of a group are in registers at once, so the group size
+        // sets this phase's register peak: FP32 groups of four fields cost the wave N=7
+        // kernel two waves per SIMD against groups of two (84 -> 71 VGPRs, 727 -> 667 us),
+        // and groups of one (70 VGPRs, 6 waves) did not speed up Maxwell N=9
+        constexpr int G = (F % 2 == 0) ? 2 : 1;
         const T *xs = sst + __mul24(el, NP * RS);
 #pragma unroll
         for (int g0 = 0; g0 < F; g0 += G) {
