@@ -191,9 +191,9 @@ int bb_solver_graph_launch(bb_solver *s);
  * chain buffers, per-step accumulation -- smaller LDS; 2, mixed precision only: as 0 with
  * L0 on the matrix cores, FP32 MFMA with FP32 accumulation; 3: as 0 with each face chain
  * owned by one wave, no workgroup barrier between flux, L0 and the chain; 4: as 3 with the
- * index tables read from global memory instead of LDS; 5: as 4 for flux and L0, then one
- * wave runs the chains of all faces with each step's items packed into its lanes).
- * bb_solver_autotune times every
+ * index tables read from global memory instead of LDS; 5: as 4 until a step's output fits
+ * half a wave, then, after one workgroup barrier, each wave runs two chains in the two
+ * halves of its lanes).  bb_solver_autotune times every
  * compiled tiling on the current state (which it leaves unchanged) and keeps the fastest. */
 int bb_solver_autotune(bb_solver *s, int prec, int *tiling);
 int bb_solver_set_tuning(bb_solver *s, int prec, int tiling);
