@@ -118,13 +118,15 @@ def test_autotune_and_tiling_invariance(ctx):
         assert rel(s.rhs(path, prec), ref) <= tol
 
 
-@pytest.mark.parametrize("pde,N,n", [("wave", 2, 3), ("wave", 7, 2), ("wave", 10, 2), ("maxwell", 5, 3),
-                                     ("maxwell", 9, 2), ("maxwell", 8, 2)])
+@pytest.mark.parametrize("pde,N,n", [("wave", 2, 3), ("wave", 3, 3), ("wave", 7, 2), ("wave", 10, 2),
+                                     ("maxwell", 5, 3), ("maxwell", 9, 2), ("maxwell", 8, 2)])
 def test_wave_owned_chains(ctx, pde, N, n):
     """chains owned by single waves (no workgroup barrier between flux, L0 and the chain),
     index tables in LDS or in global memory: FP64 bit-identical to the gather organisation, mixed within its tolerance, for every
     compiled elements-per-workgroup count (partial last workgroups included), RHS and two
     steps"""
+    # N = 2: the flux has its own buffer; N = 3: each chain's flux fills exactly the rest of
+    # its chain region (2 Nfp = Np); N >= 5: the flux sits inside the chain region
     import bbdg
 
     m, q0, s, ma, O = setup(pde, N, n=n)
