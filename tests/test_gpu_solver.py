@@ -119,7 +119,7 @@ def test_autotune_and_tiling_invariance(ctx):
 
 
 @pytest.mark.parametrize("pde,N,n", [("wave", 2, 3), ("wave", 3, 3), ("wave", 7, 2), ("wave", 10, 2),
-                                     ("maxwell", 5, 3), ("maxwell", 9, 2), ("maxwell", 8, 2)])
+                                     ("maxwell", 3, 3), ("maxwell", 5, 3), ("maxwell", 9, 2), ("maxwell", 8, 2)])
 def test_wave_owned_chains(ctx, pde, N, n):
     """chains owned by single waves (no workgroup barrier between flux, L0 and the chain),
     index tables in LDS or in global memory: FP64 bit-identical to the gather organisation, mixed within its tolerance, for every
