@@ -297,7 +297,7 @@ def face_gather_(uM: DeviceArray, N: int, q: DeviceArray):
 def interface_flux_(flux: DeviceArray, pde: str, N: int, uM: DeviceArray, mapP: DeviceArray, nx: DeviceArray,
                     ny: DeviceArray, nz: DeviceArray, bc: DeviceArray | None = None, bc_sign=None,
                     flux_kind: str = "reference"):
-    """The interface-flux loop of examples/3d_wave.jl:43-56 / 3d_maxwell.jl:187-209 (FP64)."""
+    """The interface-flux loop of examples/3d_wave.jl:43-56 / 3d_maxwell.jl:43-65 (FP64)."""
     ctx = uM.ctx
     p = {"wave": B.BB_PDE_WAVE, "maxwell": B.BB_PDE_MAXWELL}[pde]
     F = 4 if p == B.BB_PDE_WAVE else 6

@@ -1,7 +1,7 @@
 """Device-resident wave / Maxwell DG solver over the C ABI.
 
 Replaces the rhs_matvec! + ODEProblem pair of examples/3d_wave.jl:36-134 and
-examples/3d_maxwell.jl:180-282.  Two right-hand-side paths:
+examples/3d_maxwell.jl:36-138.  Two right-hand-side paths:
 
 * ``path="plain"``: one kernel per operator in the reference's order (face gather,
   flux, fx/fy/fz, nine derivative matvecs, lift, combine + /J), FP64;

@@ -5,7 +5,7 @@
 // Maxwell (examples/3d_maxwell.jl:20-33): q = (Hx,Hy,Hz,Ex,Ey,Ez), fx = (0,Ez,-Ey,0,-Hz,Hy),
 //                                        fy = (-Ez,0,Ex,Hz,0,-Hx), fz = (Ey,-Ex,0,-Hy,Hx,0);
 //                                        du = LIFT*flux + sum geo*D f
-// Both then divide by J (examples/3d_wave.jl:83, 3d_maxwell.jl:236).
+// Both then divide by J (examples/3d_wave.jl:83, 3d_maxwell.jl:92).
 #pragma once
 #include "bb_common.hpp"
 
@@ -73,7 +73,7 @@ template <> struct Pde<BB_PDE_MAXWELL> {
         default: vol[0] -= gy; vol[1] += gx; break; // Ez
         }
     }
-    // examples/3d_maxwell.jl:189-207 (kind 1: penalties d -> Jf d, (n.d)n -> (n.d)n/Jf)
+    // examples/3d_maxwell.jl:45-63 (kind 1: penalties d -> Jf d, (n.d)n -> (n.d)n/Jf)
     // 1/2 (fx(d) nx + fy(d) ny + fz(d) nz + d - pen): the two non-zero entries of
     // fx nx + fy ny + fz nz per field (n x E and -n x H), as explicit fma; the factor 1/2 is
     // applied last (exact)

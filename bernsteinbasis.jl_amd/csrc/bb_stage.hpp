@@ -3,7 +3,7 @@
 //   trace gather + interface flux -> 4-face lift (L0 + reduction chain) -> volume
 //   derivatives + geometric combine -> / J -> res = a res + dt rhs -> q_out = q_in + b res
 //
-// i.e. examples/3d_wave.jl:36-84 (or 3d_maxwell.jl:180-237) followed by one stage of a
+// i.e. examples/3d_wave.jl:36-84 (or 3d_maxwell.jl:36-93) followed by one stage of a
 // five-stage 2N-storage RK45, for a block of EB whole elements per workgroup.
 //
 // Data in HBM (per element, all contiguous, element-major):

@@ -3,7 +3,7 @@
  *
  * The entry points replace the Julia method table that BernsteinBasis.jl exports
  * (src/BernsteinBasis.jl:19-23) and the right-hand side that the examples assemble
- * from it (examples/3d_wave.jl:36-84, examples/3d_maxwell.jl:180-237).  Each
+ * from it (examples/3d_wave.jl:36-84, examples/3d_maxwell.jl:36-93).  Each
  * declaration names the reference interface it replaces.
  *
  * Conventions
@@ -149,7 +149,7 @@ int bb_elevate_apply(bb_ctx *ctx, int N, int F, int64_t K, int dtype, const void
 int bb_face_gather(bb_ctx *ctx, int N, int F, int64_t K, int dtype, const void *q, bb_layout lq, void *uM,
                    bb_layout lm);
 
-/* interface flux loop, examples/3d_wave.jl:43-56 (wave) / 3d_maxwell.jl:187-209 (Maxwell).
+/* interface flux loop, examples/3d_wave.jl:43-56 (wave) / 3d_maxwell.jl:43-65 (Maxwell).
  * uM, flux: 4*Nfp face values per element x F fields; mapP: [K][4Nfp] indices into the
  * flattened face-node array (StartUpDG md.mapP, 0-based); bc: [K][4Nfp] 0 = interior,
  * 1 = wall (uP = bc_sign .* uM, may be NULL); nxJ/nyJ/nzJ: [K][4Nfp] FP64. FP64 only. */
@@ -160,7 +160,7 @@ int bb_interface_flux(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, con
 /* ------------------------------------------------------------------ solver
  * Device-resident mesh + state for the wave (F=4: p,u,v,w) or Maxwell (F=6: Hx,Hy,Hz,
  * Ex,Ey,Ez) system on K affine tets of degree N (1 <= N <= 10).  Replaces the
- * rhs_matvec! + ODEProblem pair of examples/3d_wave.jl:36-134 / 3d_maxwell.jl:180-282
+ * rhs_matvec! + ODEProblem pair of examples/3d_wave.jl:36-134 / 3d_maxwell.jl:36-138
  * with a five-stage low-storage RK45 (Carpenter & Kennedy 1994).
  *
  * Host arrays:  geo [K][9], J [K], nJ [K][4][3] (nxJ nyJ nzJ per face, faces f1..f4),

@@ -205,7 +205,7 @@ class MeshArrays:
 
 
 def rhs(pde, N, q_aos, ma: MeshArrays, nthreads=1, flux_kind=1):
-    """q_aos: (K, Np, F) -> du (K, Np, F)  (examples/3d_wave.jl:36-84 / 3d_maxwell.jl:180-237)"""
+    """q_aos: (K, Np, F) -> du (K, Np, F)  (examples/3d_wave.jl:36-84 / 3d_maxwell.jl:36-93)"""
     q = np.ascontiguousarray(q_aos, dtype=np.float64)
     du = np.zeros_like(q)
     lib().orc_rhs(pde, flux_kind, N, ma.K, q.reshape(-1), ma.mapP.reshape(-1), ma.bc.reshape(-1), ma.bc_sign,

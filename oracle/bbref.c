@@ -434,7 +434,7 @@ static void mx_fx(const double *q, double *o) { o[0] = 0; o[1] = q[5]; o[2] = -q
 static void mx_fy(const double *q, double *o) { o[0] = -q[5]; o[1] = 0; o[2] = q[3]; o[3] = q[2]; o[4] = 0; o[5] = -q[0]; }
 static void mx_fz(const double *q, double *o) { o[0] = q[4]; o[1] = -q[3]; o[2] = 0; o[3] = -q[1]; o[4] = q[0]; o[5] = 0; }
 
-/* examples/3d_wave.jl:45-54 (wave) and examples/3d_maxwell.jl:189-207 (maxwell):
+/* examples/3d_wave.jl:45-54 (wave) and examples/3d_maxwell.jl:45-63 (maxwell):
  * the interface flux at one face node, from the jump d = uP - uM.
  * kind 0 restates the reference verbatim: its penalty terms carry Jf^0 (d_p, d) and
  * Jf^2 ((n.d) nJ) where nJ = n Jf.  kind 1 is the same upwind flux with every term
@@ -481,7 +481,7 @@ void orc_face_gather(int N, int F, int64_t K, const double *q, double *uM) {
     free(fm);
 }
 
-/* examples/3d_wave.jl:43-56 / 3d_maxwell.jl:187-209: interface flux over all face nodes */
+/* examples/3d_wave.jl:43-56 / 3d_maxwell.jl:43-65: interface flux over all face nodes */
 void orc_interface_flux(const orc_mesh_t *M, const double *uM, double *flux) {
     int F = M->F, nfp = orc_nfp(M->N);
     int64_t nf = M->K * 4 * nfp;
@@ -499,8 +499,8 @@ void orc_interface_flux(const orc_mesh_t *M, const double *uM, double *flux) {
     }
 }
 
-/* per-element part of examples/3d_wave.jl:58-80 (wave) / 3d_maxwell.jl:211-233 (maxwell),
- * then du ./= J[e] (:83 / :236), on one element of degree N: qe, due [Np][F], fe [4Nfp][F],
+/* per-element part of examples/3d_wave.jl:58-80 (wave) / 3d_maxwell.jl:67-89 (maxwell),
+ * then du ./= J[e] (:83 / :92), on one element of degree N: qe, due [Np][F], fe [4Nfp][F],
  * g the element's 9 geometric factors.  scratch: 12*Np*F + lift scratch doubles. */
 static void rhs_element_core(int pde, int N, int F, const double *qe, const double *fe, double *due,
                              const double *g, double Je, double *scratch) {
@@ -557,7 +557,7 @@ static int rhs_scratch_len(int N, int F) {
     return 12 * orc_np(N) * F + orc_lift_scratch_len(N) + 4 * orc_nfp(N) + orc_np(N);
 }
 
-/* examples/3d_wave.jl:36-84 / examples/3d_maxwell.jl:180-237 : rhs_matvec!(du, q, params, t)
+/* examples/3d_wave.jl:36-84 / examples/3d_maxwell.jl:36-93 : rhs_matvec!(du, q, params, t)
  * nthreads > 1 runs the per-element loop as examples/3d_advec_multithreaded.jl:31 does
  * (Threads.@threads over elements, per-thread scratch as MultithreadedBernsteinLift,
  * BernsteinLift.jl:40-56), with a static schedule. */
@@ -722,7 +722,7 @@ void orc_lsrk_step(int pde, int flux_kind, int N, int64_t K, double *q, double *
     free(du);
 }
 
-/* the interface-flux loop alone (examples/3d_wave.jl:43-56 / 3d_maxwell.jl:187-209),
+/* the interface-flux loop alone (examples/3d_wave.jl:43-56 / 3d_maxwell.jl:43-65),
  * uM and flux in [K][4Nfp][F] */
 void orc_flux(int pde, int flux_kind, int N, int64_t K, const double *uM, const int32_t *mapP, const int8_t *bc,
               const double *bc_sign, const double *nxJ, const double *nyJ, const double *nzJ, double *flux) {

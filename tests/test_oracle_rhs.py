@@ -1,4 +1,4 @@
-"""Pins of the oracle's RHS (examples/3d_wave.jl:36-84, 3d_maxwell.jl:180-237) and of the
+"""Pins of the oracle's RHS (examples/3d_wave.jl:36-84, 3d_maxwell.jl:36-93) and of the
 LSRK45 integrator.  The reference has no tests for these: they are pinned by invariants,
 exact solutions of the PDEs and the published integrator coefficients."""
 from fractions import Fraction
@@ -34,7 +34,7 @@ def test_constant_state_is_steady(orc, pde, kind):
 def test_linear_fields_exact_gradients(orc):
     """continuous linear fields: no jumps, so du is the exact flux divergence.
     wave: p = x  ->  du/dt = -dp/dx = -1.  Maxwell: Ez = x -> dHy/dt = dEz/dx = 1
-    (the reference's Maxwell RHS adds +sum geo*D f, examples/3d_maxwell.jl:228-231)."""
+    (the reference's Maxwell RHS adds +sum geo*D f, examples/3d_maxwell.jl:84-87)."""
     from bbdg import basis
 
     N = 3
@@ -78,7 +78,7 @@ def wave_mode(x, y, z, t):
 
 
 def maxwell_mode(x, y, z, t):
-    """TM110 cavity mode with PEC walls (the analytic solution of examples/3d_maxwell.jl:288-294)"""
+    """TM110 cavity mode with PEC walls (the analytic solution of examples/3d_maxwell.jl:144-150)"""
     w = np.pi * np.sqrt(2)
     z0 = 0 * x
     return np.stack([-np.sin(np.pi * x) * np.cos(np.pi * y) * np.sin(w * t) / np.sqrt(2),
