@@ -423,25 +423,53 @@ def batch_bench(ctx, steps=100, n_inst=64, seed=5):
     return out
 
 
-def cpu_baseline(n_sample, steps, threads):
+def cpu_threads():
+    """every core this process may run on (sched_getaffinity), capped by OMP_NUM_THREADS
+    where the environment sets it (the host's CPU share per GPU)"""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS")
+    n = avail
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return n, avail, cap
+
+
+def cpu_baseline(n_sample, steps):
+    """The reference's threaded CPU path (examples/3d_advec_multithreaded.jl:31-44 applied to
+    the Maxwell RHS of examples/3d_maxwell.jl:36-93, LSRK45), as the oracle's C restatement
+    compiled -O3 -march=native for this host, OpenMP static schedule over elements."""
     import oracle as O
     from bbdg import basis, mesh
 
+    threads, avail, cap = cpu_threads()
+    t0 = time.perf_counter()
+    L = O.baseline_lib()
+    t_build = time.perf_counter() - t0
     N = 9
     m = mesh.kuhn_mesh(n_sample, 0.1, 3, periodic=False)
     ma = O.MeshArrays(m, N, [1, 1, 1, -1, -1, -1])
     q = np.ascontiguousarray(basis.interpolate(m, N, maxwell_ic))
     dt = 0.5 * m.h / N ** 2
-    O.lsrk_step(1, N, q, dt, ma, nthreads=threads)  # warm
+    O.lsrk_step(1, N, q, dt, ma, nthreads=threads, L=L)  # warm
     t = time.perf_counter()
     for _ in range(steps):
-        O.lsrk_step(1, N, q, dt, ma, nthreads=threads)
+        O.lsrk_step(1, N, q, dt, ma, nthreads=threads, L=L)
     el = time.perf_counter() - t
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        O.rhs(1, N, q, ma, nthreads=threads, L=L)
+    el_rhs = (time.perf_counter() - t) / reps
     dofs = m.K * 220 * 6
     return {"value": dofs * steps / el, "unit": "DOF-updates/s per step", "cores": threads, "kind": "port",
-            "sample": f"{steps} LSRK45 steps of Maxwell N=9 FP64 on the {n_sample}^3x6 (K={m.K}) jittered "
-                      f"PEC mesh, oracle C restatement of the reference RHS, OpenMP static schedule, "
-                      f"{threads} threads, {el:.1f} s"}
+            "per_rhs_value": dofs / el_rhs, "ms_per_step": el / steps * 1e3,
+            "nproc": os.cpu_count(), "affinity_cores": avail, "omp_num_threads_env": cap,
+            "cpu_model": O.cpu_model(), "flags": " ".join(O.BASELINE_FLAGS + ["-fopenmp"]),
+            "schedule": "static", "K": m.K, "N": N, "build_s": t_build,
+            "sample": f"{steps} LSRK45 steps (+{reps} RHS evaluations) of Maxwell N=9 FP64 on the {n_sample}^3x6 "
+                      f"(K={m.K}) jittered PEC mesh: the oracle's C restatement of the reference's threaded RHS, "
+                      f"gcc {' '.join(O.BASELINE_FLAGS)} on this host, OpenMP static schedule, {threads} threads "
+                      f"({avail} cores in the affinity mask, OMP_NUM_THREADS={cap}), {el:.1f} s"}
 
 
 def main():
@@ -569,8 +597,7 @@ def main():
         batch = batch_bench(bbdg.default_context(), steps=args.batch_steps)
     cpu = None
     if not args.no_cpu:
-        thr = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(args.cpu_n, args.cpu_steps, thr)
+        cpu = cpu_baseline(args.cpu_n, args.cpu_steps)
         log(f"cpu baseline: {cpu['value']:.3e} DOF/s ({cpu['sample']})")
     out = {
         "metric": "DOF-updates/s per RK step at N=7-9: FP32+FP64-lift batched vs plain FP64",

@@ -5,11 +5,18 @@ TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
 (or as the timed CPU baseline).  The product library never calls it.
 
 ``build()`` compiles ``bbref.c`` with gcc into ``oracle/_build/liboracle.so``; ``lib()``
-loads it (building first only when nothing is built).
+loads it (building first only when nothing is built).  The checker is built without
+floating-point contraction (-ffp-contract=off), so every product and sum rounds as written.
+
+``baseline_lib()`` is the same source built the way a CPU baseline of the reference's
+threaded path is built: -O3 -march=native (contraction allowed), OpenMP.  -march=native
+targets the machine that compiles it, so this variant is compiled on first use on the
+machine that runs it, into a file keyed by that machine's CPU model and the flags.
 """
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
 import os
 import subprocess
 
@@ -18,18 +25,53 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD_DIR = os.path.join(HERE, "_build")
 SO = os.path.join(BUILD_DIR, "liboracle.so")
+SRC = os.path.join(HERE, "bbref.c")
+CHECK_FLAGS = ["-O2", "-ffp-contract=off"]
+# (ISO C mode turns contraction off by default; the baseline asks for it explicitly)
+BASELINE_FLAGS = ["-O3", "-march=native", "-ffp-contract=fast"]
 
 _lib = None
+_baseline = None
+
+
+def _compile(out: str, flags: list, extra: list | None = None) -> str:
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    tmp = f"{out}.tmp{os.getpid()}"
+    cmd = ["gcc", *flags, "-fPIC", "-shared", "-fopenmp", "-std=c11", *(extra or []), "-o", tmp, SRC, "-lm"]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, out)
+    return out
 
 
 def build() -> str:
-    os.makedirs(BUILD_DIR, exist_ok=True)
-    src = os.path.join(HERE, "bbref.c")
-    tmp = SO + ".tmp"
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-std=c11", "-o", tmp, src, "-lm"]
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, SO)
-    return SO
+    return _compile(SO, CHECK_FLAGS)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def baseline_path() -> str:
+    key = hashlib.sha1((cpu_model() + " ".join(BASELINE_FLAGS)).encode()).hexdigest()[:12]
+    return os.path.join(BUILD_DIR, f"liboracle_baseline_{key}.so")
+
+
+def baseline_lib():
+    """the oracle compiled -O3 -march=native for this machine (timed CPU baseline only)"""
+    global _baseline
+    if _baseline is None:
+        path = baseline_path()
+        if not os.path.exists(path):
+            _compile(path, BASELINE_FLAGS)
+        _baseline = _bind(C.CDLL(path))
+    return _baseline
 
 
 def lib():
@@ -37,49 +79,52 @@ def lib():
     if _lib is None:
         if not os.path.exists(SO):
             build()
-        L = C.CDLL(SO)
-        i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
-        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
-        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
-        L.orc_np.argtypes = [C.c_int]
-        L.orc_nfp.argtypes = [C.c_int]
-        L.orc_tri_offsets.argtypes = [C.c_int, i64p]
-        L.orc_tet_offsets.argtypes = [C.c_int, i64p]
-        L.orc_ij_to_linear.argtypes = [C.c_int, C.c_int, i64p]
-        L.orc_ij_to_linear.restype = C.c_int64
-        L.orc_ijk_to_linear.argtypes = [C.c_int, C.c_int, C.c_int, i64p, i64p]
-        L.orc_ijk_to_linear.restype = C.c_int64
-        L.orc_linear_to_ijkl.argtypes = [C.c_int, i32p]
-        L.orc_linear_to_ijk_2d.argtypes = [C.c_int, i32p]
-        L.orc_l_j.argtypes = [C.c_int, dp]
-        L.orc_deriv_entry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
-        L.orc_deriv_entry.restype = C.c_double
-        L.orc_deriv_mul.argtypes = [C.c_int, C.c_int, dp, dp]
-        L.orc_deriv_mul_cols.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp]
-        L.orc_elev_entry.argtypes = [C.c_int, C.c_int, C.c_int]
-        L.orc_elev_entry.restype = C.c_double
-        L.orc_elevation_multiply.argtypes = [C.c_int, dp, dp]
-        L.orc_reduction_multiply.argtypes = [C.c_int, dp, dp]
-        L.orc_face_lift.argtypes = [C.c_int, dp, dp, dp, dp]
-        L.orc_lift.argtypes = [C.c_int, dp, dp, dp]
-        L.orc_lift_scratch_len.argtypes = [C.c_int]
-        L.orc_lift_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
-        L.orc_face_lift_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
-        L.orc_fmask.argtypes = [C.c_int, i32p]
-        L.orc_face_gather.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp]
-        i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
-        L.orc_rhs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, i32p, i8p, dp, dp, dp, dp, dp, dp, dp, C.c_int]
-        L.orc_lsrk_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, dp, C.c_double, i32p, i8p, dp, dp, dp, dp,
-                                    dp, dp, C.c_int]
-        L.orc_lsrk_coeffs.argtypes = [dp, dp]
-        L.orc_face_lift_ld_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
-        L.orc_flux.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, i32p, i8p, dp, dp, dp, dp, dp]
-        L.orc_rhs_md.argtypes = [C.c_int, C.c_int, C.c_int64, i32p, i64p, dp, i32p, i32p, i32p, i32p, dp, dp, dp, dp,
-                                 dp, i64p, dp, C.c_int]
-        L.orc_lsrk_step_md.argtypes = [C.c_int, C.c_int, C.c_int64, i32p, i64p, dp, dp, dp, i32p, i32p, i32p, i32p,
-                                       dp, dp, dp, dp, dp, i64p, C.c_int]
-        _lib = L
+        _lib = _bind(C.CDLL(SO))
     return _lib
+
+
+def _bind(L):
+    i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+    i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    L.orc_np.argtypes = [C.c_int]
+    L.orc_nfp.argtypes = [C.c_int]
+    L.orc_tri_offsets.argtypes = [C.c_int, i64p]
+    L.orc_tet_offsets.argtypes = [C.c_int, i64p]
+    L.orc_ij_to_linear.argtypes = [C.c_int, C.c_int, i64p]
+    L.orc_ij_to_linear.restype = C.c_int64
+    L.orc_ijk_to_linear.argtypes = [C.c_int, C.c_int, C.c_int, i64p, i64p]
+    L.orc_ijk_to_linear.restype = C.c_int64
+    L.orc_linear_to_ijkl.argtypes = [C.c_int, i32p]
+    L.orc_linear_to_ijk_2d.argtypes = [C.c_int, i32p]
+    L.orc_l_j.argtypes = [C.c_int, dp]
+    L.orc_deriv_entry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    L.orc_deriv_entry.restype = C.c_double
+    L.orc_deriv_mul.argtypes = [C.c_int, C.c_int, dp, dp]
+    L.orc_deriv_mul_cols.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp]
+    L.orc_elev_entry.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.orc_elev_entry.restype = C.c_double
+    L.orc_elevation_multiply.argtypes = [C.c_int, dp, dp]
+    L.orc_reduction_multiply.argtypes = [C.c_int, dp, dp]
+    L.orc_face_lift.argtypes = [C.c_int, dp, dp, dp, dp]
+    L.orc_lift.argtypes = [C.c_int, dp, dp, dp]
+    L.orc_lift_scratch_len.argtypes = [C.c_int]
+    L.orc_lift_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
+    L.orc_face_lift_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
+    L.orc_fmask.argtypes = [C.c_int, i32p]
+    L.orc_face_gather.argtypes = [C.c_int, C.c_int, C.c_int64, dp, dp]
+    i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+    L.orc_rhs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, i32p, i8p, dp, dp, dp, dp, dp, dp, dp, C.c_int]
+    L.orc_lsrk_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, dp, C.c_double, i32p, i8p, dp, dp, dp, dp,
+                                dp, dp, C.c_int]
+    L.orc_lsrk_coeffs.argtypes = [dp, dp]
+    L.orc_face_lift_ld_cols.argtypes = [C.c_int, C.c_int64, dp, dp]
+    L.orc_flux.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, dp, i32p, i8p, dp, dp, dp, dp, dp]
+    L.orc_rhs_md.argtypes = [C.c_int, C.c_int, C.c_int64, i32p, i64p, dp, i32p, i32p, i32p, i32p, dp, dp, dp, dp,
+                             dp, i64p, dp, C.c_int]
+    L.orc_lsrk_step_md.argtypes = [C.c_int, C.c_int, C.c_int64, i32p, i64p, dp, dp, dp, i32p, i32p, i32p, i32p,
+                                   dp, dp, dp, dp, dp, i64p, C.c_int]
+    return L
 
 
 # ----------------------------------------------------------------- thin wrappers
@@ -204,11 +249,12 @@ class MeshArrays:
         self.bc_sign = np.ascontiguousarray(np.asarray(bc_sign, dtype=np.float64))
 
 
-def rhs(pde, N, q_aos, ma: MeshArrays, nthreads=1, flux_kind=1):
-    """q_aos: (K, Np, F) -> du (K, Np, F)  (examples/3d_wave.jl:36-84 / 3d_maxwell.jl:36-93)"""
+def rhs(pde, N, q_aos, ma: MeshArrays, nthreads=1, flux_kind=1, L=None):
+    """q_aos: (K, Np, F) -> du (K, Np, F)  (examples/3d_wave.jl:36-84 / 3d_maxwell.jl:36-93).
+    L: the loaded oracle to run (default the checker build; baseline_lib() for timing)"""
     q = np.ascontiguousarray(q_aos, dtype=np.float64)
     du = np.zeros_like(q)
-    lib().orc_rhs(pde, flux_kind, N, ma.K, q.reshape(-1), ma.mapP.reshape(-1), ma.bc.reshape(-1), ma.bc_sign,
+    (L or lib()).orc_rhs(pde, flux_kind, N, ma.K, q.reshape(-1), ma.mapP.reshape(-1), ma.bc.reshape(-1), ma.bc_sign,
                   ma.nx.reshape(-1), ma.ny.reshape(-1), ma.nz.reshape(-1), ma.geo.reshape(-1), ma.J, du.reshape(-1),
                   nthreads)
     return du
@@ -232,11 +278,11 @@ def flux(pde, N, uM, ma: MeshArrays, flux_kind=1):
     return out
 
 
-def lsrk_step(pde, N, q_aos, dt, ma: MeshArrays, nthreads=1, flux_kind=1):
-    """in-place LSRK45 step on q_aos (K, Np, F) float64 C-contiguous"""
+def lsrk_step(pde, N, q_aos, dt, ma: MeshArrays, nthreads=1, flux_kind=1, L=None):
+    """in-place LSRK45 step on q_aos (K, Np, F) float64 C-contiguous (L as for rhs)"""
     assert q_aos.flags.c_contiguous and q_aos.dtype == np.float64
     res = np.zeros_like(q_aos)
-    lib().orc_lsrk_step(pde, flux_kind, N, ma.K, q_aos.reshape(-1), res.reshape(-1), dt, ma.mapP.reshape(-1),
+    (L or lib()).orc_lsrk_step(pde, flux_kind, N, ma.K, q_aos.reshape(-1), res.reshape(-1), dt, ma.mapP.reshape(-1),
                         ma.bc.reshape(-1), ma.bc_sign, ma.nx.reshape(-1), ma.ny.reshape(-1), ma.nz.reshape(-1),
                         ma.geo.reshape(-1), ma.J, nthreads)
     return q_aos
