@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 SO = os.path.join(os.path.dirname(HERE), "libbbdg.so")
 
-BB_OK, BB_ERR_ARG, BB_ERR_NULL, BB_ERR_HIP, BB_ERR_UNSUPPORTED, BB_ERR_STATE = range(6)
+BB_OK, BB_ERR_ARG, BB_ERR_NULL, BB_ERR_HIP, BB_ERR_UNSUPPORTED, BB_ERR_STATE, BB_ERR_NOMEM = range(7)
 BB_F64, BB_F32 = 0, 1
 BB_PDE_WAVE, BB_PDE_MAXWELL = 0, 1
 BB_FLUX_REFERENCE, BB_FLUX_UPWIND = 0, 1
@@ -19,7 +19,8 @@ BB_LIFT_PLAIN, BB_LIFT_ACC64, BB_LIFT_TWOSUM = 0, 1, 2
 BB_PREC_F64, BB_PREC_MIXED = 0, 1
 BB_PATH_PLAIN, BB_PATH_FUSED = 0, 1
 
-ERR_NAMES = {1: "BB_ERR_ARG", 2: "BB_ERR_NULL", 3: "BB_ERR_HIP", 4: "BB_ERR_UNSUPPORTED", 5: "BB_ERR_STATE"}
+ERR_NAMES = {1: "BB_ERR_ARG", 2: "BB_ERR_NULL", 3: "BB_ERR_HIP", 4: "BB_ERR_UNSUPPORTED", 5: "BB_ERR_STATE",
+             6: "BB_ERR_NOMEM"}
 
 
 class bb_layout(C.Structure):

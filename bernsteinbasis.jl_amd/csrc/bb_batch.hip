@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "bb_dispatch.hpp"
+#include "bb_host.hpp"
 #include "bb_stage.hpp"
 
 using namespace bb;
@@ -88,7 +89,7 @@ static StageArgs batch_args(bb_batch *b, const BatchGroup &g, int prec, int cur)
 static int batch_launch(bb_batch *b, const BatchGroup &g, int prec, int tiling, const StageArgs &a) {
     int rc = batch_fn(prec)(g.N, tiling, a, b->ctx->stream, false);
     if (rc == BB_ERR_UNSUPPORTED) return bb_fail(b->ctx, rc, "no fused kernel for N=%d with tiling %d", g.N, tiling);
-    if (rc) return bb_fail(b->ctx, rc, "fused stage launch failed: %s", hipGetErrorString(hipGetLastError()));
+    if (rc) return bb_fail(b->ctx, rc, "fused stage launch failed: %s", hipGetErrorString(t_launch_err));
     return BB_OK;
 }
 
@@ -156,142 +157,83 @@ static void batch_drop_graphs(bb_batch *b) {
 
 extern "C" {
 
-int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_t *deg_host, const double *geo_host,
-                    const double *J_host, const double *nJ_host, const int32_t *nbr_host,
-                    const int32_t *nbr_face_host, const int32_t *perm_host, const int32_t *bc_host,
-                    const double *bc_sign_host, const double *dt_host, const double *transfer_host,
-                    const int64_t *transfer_off_host, int64_t transfer_len, bb_batch **out) {
-    if (!ctx || !out) return BB_ERR_NULL;
-    *out = nullptr;
-    if (pde != BB_PDE_WAVE) return bb_fail(ctx, BB_ERR_UNSUPPORTED, "batches of mixed degree: wave system only");
-    if (flux_kind != BB_FLUX_REFERENCE && flux_kind != BB_FLUX_UPWIND) return bb_fail(ctx, BB_ERR_ARG, "flux %d", flux_kind);
-    if (K < 1) return bb_fail(ctx, BB_ERR_ARG, "K must be >= 1");
-    if (!deg_host || !geo_host || !J_host || !nJ_host || !nbr_host || !nbr_face_host || !perm_host || !bc_host ||
-        !dt_host || !transfer_host || !transfer_off_host)
-        return bb_fail(ctx, BB_ERR_NULL, "null batch array");
-    const int F = 4;
-    std::vector<int64_t> qoff_e(K);
-    int64_t words = 0;
-    for (int64_t e = 0; e < K; ++e) {
-        const int N = deg_host[e];
-        if (N < 1 || N > kStageMaxN) return bb_fail(ctx, BB_ERR_ARG, "degree %d of element %lld", N, (long long)e);
-        if (e && N < deg_host[e - 1]) return bb_fail(ctx, BB_ERR_ARG, "elements must be grouped by increasing degree");
-        if (!(J_host[e] > 0.0)) return bb_fail(ctx, BB_ERR_ARG, "non-positive Jacobian at element %lld", (long long)e);
-        qoff_e[e] = words;
-        words += (int64_t)F * np3(N);
+static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_t *deg_host,
+                        const double *geo_host, const double *J_host, const double *nJ_host,
+                        const int32_t *nbr_host, const int32_t *nbr_face_host, const int32_t *perm_host,
+                        const int32_t *bc_host, const double *bc_sign_host, const double *dt_host,
+                        const double *transfer_host, const int64_t *transfer_off_host, int64_t transfer_len,
+                        bb_batch **out) {
+    BatchHost h;
+    int rc = batch_host_setup(ctx, pde, flux_kind, K, deg_host, geo_host, J_host, nJ_host, nbr_host, nbr_face_host,
+                              perm_host, bc_host, dt_host, transfer_host, transfer_off_host, transfer_len, h);
+    if (rc) return rc;
+    // the per-degree index tables, before anything is allocated on the device
+    std::vector<std::vector<uint16_t>> tabs;
+    std::vector<std::vector<float>> l0as;
+    for (const auto &hg : h.groups) {
+        tabs.push_back(stage_tables(hg.N));
+        l0as.push_back(l0_mfma_table(hg.N));
     }
-    std::vector<int2> conn(K * 4);
-    std::vector<int64_t> nboff(K * 4, 0);
-    std::vector<double> nj5(K * 20);
-    for (int64_t t = 0; t < K * 4; ++t) {
-        const int64_t e = t / 4;
-        const int nb = nbr_host[t], nf = nbr_face_host[t], p = perm_host[t], bc = bc_host[t];
-        if (nb < 0 || nb >= K || nf < 0 || nf > 3 || p < 0 || p > 5 || bc < 0 || bc > 1)
-            return bb_fail(ctx, BB_ERR_ARG, "bad connectivity at element %lld face %d", (long long)e, (int)(t % 4));
-        const int N = deg_host[e], Nn = deg_host[nb];
-        if (!bc && Nn != N) {
-            const int64_t o = transfer_off_host[11 * N + Nn];
-            if (o < 0 || o + (int64_t)nfp2(N) * nfp2(Nn) > transfer_len)
-                return bb_fail(ctx, BB_ERR_ARG, "no transfer table from degree %d to %d", Nn, N);
-        }
-        conn[t] = make_int2(nb, nf | (p << 2) | (bc << 5) | (Nn << 8));
-        nboff[t] = bc ? 0 : qoff_e[nb];
-        const double *n = nJ_host + t * 3;
-        const double Jf = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-        if (!(Jf > 0.0)) return bb_fail(ctx, BB_ERR_ARG, "zero face normal at element %lld", (long long)e);
-        double *o = nj5.data() + t * 5;
-        o[0] = n[0];
-        o[1] = n[1];
-        o[2] = n[2];
-        o[3] = Jf;
-        o[4] = 1.0 / Jf;
+    std::vector<BatchGroup> groups(h.groups.size());
+    for (size_t i = 0; i < groups.size(); ++i) {
+        groups[i].N = h.groups[i].N;
+        groups[i].start = h.groups[i].start;
+        groups[i].count = h.groups[i].count;
+        groups[i].qoff = h.groups[i].qoff;
     }
-    bb_batch *b = new bb_batch();
+    bb_batch *b = new (std::nothrow) bb_batch();
+    if (!b) return bb_fail(ctx, BB_ERR_NOMEM, "host memory allocation failed");
     b->ctx = ctx;
     b->pde = pde;
-    b->F = F;
+    b->F = h.F;
     b->flux_kind = flux_kind;
     b->K = K;
-    b->words = words;
+    b->words = h.words;
+    b->groups.swap(groups);
     if (bc_sign_host)
-        for (int f = 0; f < F; ++f) b->sign[f] = bc_sign_host[f];
-    for (int64_t e = 0; e < K;) {
-        BatchGroup g;
-        g.N = deg_host[e];
-        g.start = e;
-        g.qoff = qoff_e[e];
-        while (e < K && deg_host[e] == g.N) ++e;
-        g.count = e - g.start;
-        b->groups.push_back(g);
-    }
-    // elevations (Nn < N) as sparse rows; every table also in FP32
-    std::vector<int32_t> erow;
-    std::vector<int64_t> eoff(121, -1);
-    std::vector<uint8_t> ecol;
-    std::vector<double> evd;
-    for (int N = 1; N <= kStageMaxN; ++N)
-        for (int Nn = 1; Nn < N; ++Nn) {
-            const int64_t o = transfer_off_host[11 * N + Nn];
-            if (o < 0 || o + (int64_t)nfp2(N) * nfp2(Nn) > transfer_len) continue;
-            eoff[11 * N + Nn] = (int64_t)erow.size();
-            for (int r = 0; r < nfp2(N); ++r) {
-                erow.push_back((int32_t)evd.size());
-                for (int c = 0; c < nfp2(Nn); ++c) {
-                    const double v = transfer_host[o + (int64_t)r * nfp2(Nn) + c];
-                    if (v != 0.0) {
-                        ecol.push_back((uint8_t)c);
-                        evd.push_back(v);
-                    }
-                }
-            }
-            erow.push_back((int32_t)evd.size());
-        }
-    std::vector<float> evf(evd.begin(), evd.end()), trf(transfer_host, transfer_host + transfer_len);
+        for (int f = 0; f < h.F; ++f) b->sign[f] = bc_sign_host[f];
     auto fail = [&](hipError_t err) {
         int r = bb_fail(ctx, BB_ERR_HIP, "batch allocation: %s", hipGetErrorString(err));
         bb_batch_destroy(b);
         return r;
     };
     hipError_t err;
-    std::vector<double> invJ(K), geoh(K * 9);
-    std::vector<float> geoh32(K * 9);
-    for (int64_t e = 0; e < K; ++e) invJ[e] = 1.0 / J_host[e];
-    for (int64_t t = 0; t < K * 9; ++t) {
-        geoh[t] = 0.5 * geo_host[t];
-        geoh32[t] = (float)geoh[t];
-    }
+    const int64_t words = h.words;
+    const int64_t tl = transfer_len > 0 ? transfer_len : 1;
     if ((err = hipMalloc(&b->geo, K * 9 * sizeof(double))) || (err = hipMalloc(&b->invJ, K * sizeof(double))) ||
         (err = hipMalloc(&b->geo32, K * 9 * sizeof(float))) ||
         (err = hipMalloc(&b->nJ, K * 20 * sizeof(double))) || (err = hipMalloc(&b->dte, K * sizeof(double))) ||
         (err = hipMalloc(&b->conn, K * 4 * sizeof(int2))) || (err = hipMalloc(&b->nboff, K * 4 * sizeof(int64_t))) ||
-        (err = hipMalloc(&b->tr, (transfer_len > 0 ? transfer_len : 1) * sizeof(double))) ||
+        (err = hipMalloc(&b->tr, tl * sizeof(double))) ||
         (err = hipMalloc(&b->troff, 121 * sizeof(int64_t))) || (err = hipMalloc(&b->q[0], words * sizeof(double))) ||
-        (err = hipMalloc(&b->trf, (transfer_len > 0 ? transfer_len : 1) * sizeof(float))) ||
-        (err = hipMalloc(&b->erow, (erow.size() + 1) * sizeof(int32_t))) ||
+        (err = hipMalloc(&b->trf, tl * sizeof(float))) ||
+        (err = hipMalloc(&b->erow, (h.erow.size() + 1) * sizeof(int32_t))) ||
         (err = hipMalloc(&b->eoff, 121 * sizeof(int64_t))) ||
-        (err = hipMalloc(&b->ecol, ecol.size() + 1)) || (err = hipMalloc(&b->evd, (evd.size() + 1) * sizeof(double))) ||
-        (err = hipMalloc(&b->evf, (evf.size() + 1) * sizeof(float))) ||
+        (err = hipMalloc(&b->ecol, h.ecol.size() + 1)) ||
+        (err = hipMalloc(&b->evd, (h.evd.size() + 1) * sizeof(double))) ||
+        (err = hipMalloc(&b->evf, (h.evf.size() + 1) * sizeof(float))) ||
         (err = hipMalloc(&b->q[1], words * sizeof(double))))
         return fail(err);
-    if ((err = hipMemcpy(b->geo, geoh.data(), K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->geo32, geoh32.data(), K * 9 * sizeof(float), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->invJ, invJ.data(), K * sizeof(double), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->nJ, nj5.data(), K * 20 * sizeof(double), hipMemcpyHostToDevice)) ||
+    if ((err = hipMemcpy(b->geo, h.geoh.data(), K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->geo32, h.geoh32.data(), K * 9 * sizeof(float), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->invJ, h.invJ.data(), K * sizeof(double), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->nJ, h.nj5.data(), K * 20 * sizeof(double), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->dte, dt_host, K * sizeof(double), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->conn, conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->nboff, nboff.data(), K * 4 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->conn, h.conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->nboff, h.nboff.data(), K * 4 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->tr, transfer_host, transfer_len * sizeof(double), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->troff, transfer_off_host, 121 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->trf, trf.data(), trf.size() * sizeof(float), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->erow, erow.data(), erow.size() * sizeof(int32_t), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->eoff, eoff.data(), 121 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->ecol, ecol.data(), ecol.size(), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->evd, evd.data(), evd.size() * sizeof(double), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->evf, evf.data(), evf.size() * sizeof(float), hipMemcpyHostToDevice)))
+        (err = hipMemcpy(b->trf, h.trf.data(), h.trf.size() * sizeof(float), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->erow, h.erow.data(), h.erow.size() * sizeof(int32_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->eoff, h.eoff.data(), 121 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->ecol, h.ecol.data(), h.ecol.size(), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->evd, h.evd.data(), h.evd.size() * sizeof(double), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->evf, h.evf.data(), h.evf.size() * sizeof(float), hipMemcpyHostToDevice)))
         return fail(err);
-    for (auto &g : b->groups) {
-        const std::vector<uint16_t> tab = stage_tables(g.N);
-        const std::vector<float> l0a = l0_mfma_table(g.N);
+    for (size_t i = 0; i < b->groups.size(); ++i) {
+        BatchGroup &g = b->groups[i];
+        const std::vector<uint16_t> &tab = tabs[i];
+        const std::vector<float> &l0a = l0as[i];
         if ((err = hipMalloc(&g.tab, tab.size() * sizeof(uint16_t))) ||
             (err = hipMemcpy(g.tab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice)))
             return fail(err);
@@ -299,7 +241,6 @@ int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_
                              (err = hipMemcpy(g.l0a, l0a.data(), l0a.size() * sizeof(float), hipMemcpyHostToDevice))))
             return fail(err);
     }
-    int rc;
     if ((rc = bb_fill_zero(ctx, b->q[0], words * sizeof(double))) || (rc = bb_fill_zero(ctx, b->q[1], words * sizeof(double)))) {
         bb_batch_destroy(b);
         return rc;
@@ -346,7 +287,7 @@ int bb_batch_get_state(bb_batch *b, double *q_host) {
     return BB_OK;
 }
 
-int bb_batch_rhs(bb_batch *b, int prec, double *du_host) {
+static int batch_rhs(bb_batch *b, int prec, double *du_host) {
     if (!b || !du_host) return BB_ERR_NULL;
     int rc = batch_prepare(b, prec);
     if (rc) return rc;
@@ -470,6 +411,27 @@ int bb_batch_autotune(bb_batch *b, int prec, int *tilings) {
         for (auto &g : b->groups) tilings[g.N] = g.tiling[prec];
     }
     return BB_OK;
+}
+
+// Entry points that allocate on the host run under bb_guard: an exception becomes
+// BB_ERR_NOMEM.
+int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_t *deg_host, const double *geo_host,
+                    const double *J_host, const double *nJ_host, const int32_t *nbr_host,
+                    const int32_t *nbr_face_host, const int32_t *perm_host, const int32_t *bc_host,
+                    const double *bc_sign_host, const double *dt_host, const double *transfer_host,
+                    const int64_t *transfer_off_host, int64_t transfer_len, bb_batch **out) {
+    if (!ctx || !out) return BB_ERR_NULL;
+    *out = nullptr;
+    return bb_guard(ctx, [&] {
+        return batch_create(ctx, pde, flux_kind, K, deg_host, geo_host, J_host, nJ_host, nbr_host, nbr_face_host,
+                            perm_host, bc_host, bc_sign_host, dt_host, transfer_host, transfer_off_host, transfer_len,
+                            out);
+    });
+}
+
+int bb_batch_rhs(bb_batch *b, int prec, double *du_host) {
+    if (!b || !du_host) return BB_ERR_NULL;
+    return bb_guard(b->ctx, [&] { return batch_rhs(b, prec, du_host); });
 }
 
 }  // extern "C"

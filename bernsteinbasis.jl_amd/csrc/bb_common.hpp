@@ -9,11 +9,18 @@
 //                                                   tri_N(j) - j k = tri_{N-k}(j))
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
-#include <string>
+
+#include <exception>
+#include <new>
 
 #include "bbdg.h"
+
+// largest element count any entry point accepts (24-bit record offsets and int32 face-node
+// indices of the plain path stay in range below it)
+constexpr int64_t kMaxElements = (int64_t(1) << 31) / 4;
 
 namespace bb {
 
@@ -67,11 +74,46 @@ struct bb_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool capturing = false;
     void *blas = nullptr;  // library GEMM handle, created on first use (bb_gemm)
-    std::string err;
+    char err[1024] = {0};  // message of the last failure (fixed size: recording it never allocates)
 };
 
-// error helpers (host)
-int bb_fail(bb_ctx *ctx, int code, const char *fmt, ...);
+// error helpers (host).  bb_fail formats the message into the context's fixed buffer
+// (truncating), so it cannot fail or throw.
+__attribute__((format(printf, 3, 4))) inline int bb_fail(bb_ctx *ctx, int code, const char *fmt, ...) noexcept {
+    if (ctx) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(ctx->err, sizeof ctx->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+// Runs the body of an entry point that allocates on the host.  A failed host allocation,
+// or any other C++ exception, becomes a status code: nothing throws across the C ABI.
+template <class Fn> inline int bb_guard(bb_ctx *ctx, Fn &&fn) noexcept {
+    try {
+        return fn();
+    } catch (const std::bad_alloc &) {
+        return bb_fail(ctx, BB_ERR_NOMEM, "host memory allocation failed");
+    } catch (const std::exception &e) {
+        return bb_fail(ctx, BB_ERR_NOMEM, "host-side failure: %s", e.what());
+    } catch (...) {
+        return bb_fail(ctx, BB_ERR_NOMEM, "host-side failure (unknown exception)");
+    }
+}
+
+// the HIP error of the last kernel launch made by a launcher that returns a status code,
+// per host thread (hipGetLastError clears the error, so the launcher keeps it for the
+// message its caller formats)
+namespace bb {
+inline thread_local hipError_t t_launch_err = hipSuccess;
+inline int launch_status() {
+    t_launch_err = hipGetLastError();
+    return t_launch_err == hipSuccess ? BB_OK : BB_ERR_HIP;
+}
+}  // namespace bb
+
 // releases the context's library GEMM handle (bb_dense.hip)
 void bb_blas_release(bb_ctx *ctx);
 #define BB_HIP(ctx, call)                                                                           \
@@ -83,5 +125,12 @@ void bb_blas_release(bb_ctx *ctx);
     } while (0)
 #define BB_LAUNCH_CHECK(ctx) BB_HIP(ctx, hipGetLastError())
 
-// shared argument validation
-int bb_check_common(bb_ctx *ctx, int N, int F, int64_t K, int nmin, int nmax);
+// shared argument validation: degree in [nmin, nmax], 1 <= F <= 64, 0 <= K <= 2^29
+inline int bb_check_common(bb_ctx *ctx, int N, int F, int64_t K, int nmin, int nmax) noexcept {
+    if (!ctx) return BB_ERR_NULL;
+    if (N < nmin || N > nmax) return bb_fail(ctx, BB_ERR_ARG, "degree N=%d outside [%d,%d]", N, nmin, nmax);
+    if (F < 1 || F > 64) return bb_fail(ctx, BB_ERR_ARG, "field count F=%d outside [1,64]", F);
+    if (K < 0) return bb_fail(ctx, BB_ERR_ARG, "element count K=%lld < 0", (long long)K);
+    if (K > kMaxElements) return bb_fail(ctx, BB_ERR_ARG, "element count K=%lld too large", (long long)K);
+    return BB_OK;
+}

@@ -36,7 +36,7 @@ int bb_gemm(bb_ctx *ctx, int dtype, int64_t M, int64_t N, int64_t K, double alph
     if (M == 0 || N == 0) return BB_OK;
     if (!A || !B || !C) return bb_fail(ctx, BB_ERR_NULL, "null GEMM operand");
     if (lda < M || ldb < K || ldc < M) return bb_fail(ctx, BB_ERR_ARG, "leading dimension too small");
-    if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || ldb > INT32_MAX || ldc > INT32_MAX)
+    if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || lda > INT32_MAX || ldb > INT32_MAX || ldc > INT32_MAX)
         return bb_fail(ctx, BB_ERR_ARG, "GEMM size beyond 32-bit indices");
     rocblas_handle h;
     int rc = blas_handle(ctx, &h);

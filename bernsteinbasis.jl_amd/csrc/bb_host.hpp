@@ -1,0 +1,198 @@
+// bb_host.hpp -- host-side set-up of a solver (one mesh, one degree) and of a batch of
+// meshes with per-element degrees: argument validation and every host array the device
+// copies are made from.  No HIP calls happen here, so these functions are also built
+// host-only with the address and undefined-behaviour sanitizers and exercised without a
+// GPU (their allocations may throw std::bad_alloc; the entry points that call them run
+// under bb_guard).
+#pragma once
+#include <math.h>
+
+#include <vector>
+
+#include "bb_stage.hpp"
+
+namespace bb {
+
+// connectivity word of one face: neighbour element, and face | perm << 2 | bc << 5 |
+// neighbour degree << 8
+inline int conn_word(int nf, int perm, int bc, int Nn) { return nf | (perm << 2) | (bc << 5) | (Nn << 8); }
+
+// per-face scalings nxJ nyJ nzJ Jf 1/Jf from the three components of nJ; false for a zero normal
+inline bool face_scalings(const double *n, double *o) {
+    const double Jf = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (!(Jf > 0.0)) return false;
+    o[0] = n[0];
+    o[1] = n[1];
+    o[2] = n[2];
+    o[3] = Jf;
+    o[4] = 1.0 / Jf;
+    return true;
+}
+
+struct SolverHost {
+    int F = 0, NP = 0, NFP = 0;
+    std::vector<int2> conn;                // [K][4]
+    std::vector<double> nJ3;               // [K][4][3] as given (the plain path's face normals)
+    std::vector<double> nj5;               // [K][4][5]
+    std::vector<double> invJ, geoh64;      // [K], [K][9] geo / 2
+    std::vector<float> geoh32;             // [K][9]
+    std::vector<uint16_t> tab;             // per-degree index tables of the stage kernel
+    std::vector<float> l0a;                // L0 as matrix-core A operands
+};
+
+// validates the arguments of bb_solver_create and builds every host array it uploads
+inline int solver_host_setup(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *geo,
+                             const double *J, const double *nJ, const int32_t *nbr, const int8_t *nbr_face,
+                             const int8_t *perm, const int8_t *bc, SolverHost &h) {
+    if (!ctx) return BB_ERR_NULL;
+    if (pde != BB_PDE_WAVE && pde != BB_PDE_MAXWELL) return bb_fail(ctx, BB_ERR_ARG, "pde %d", pde);
+    if (flux_kind != BB_FLUX_REFERENCE && flux_kind != BB_FLUX_UPWIND)
+        return bb_fail(ctx, BB_ERR_ARG, "flux %d", flux_kind);
+    h.F = pde == BB_PDE_WAVE ? 4 : 6;
+    int rc = bb_check_common(ctx, N, h.F, K, 1, kStageMaxN);
+    if (rc) return rc;
+    if (K < 1) return bb_fail(ctx, BB_ERR_ARG, "K must be >= 1");
+    if (!geo || !J || !nJ || !nbr || !nbr_face || !perm || !bc) return bb_fail(ctx, BB_ERR_NULL, "null mesh array");
+    h.NP = np3(N);
+    h.NFP = nfp2(N);
+    h.conn.resize(K * 4);
+    h.nj5.resize(K * 20);
+    for (int64_t t = 0; t < K * 4; ++t) {
+        const int nb = nbr[t], nf = nbr_face[t], p = perm[t], b = bc[t];
+        if (nb < 0 || nb >= K || nf < 0 || nf > 3 || p < 0 || p > 5 || b < 0 || b > 1)
+            return bb_fail(ctx, BB_ERR_ARG, "bad connectivity at element %lld face %d", (long long)(t / 4),
+                           (int)(t % 4));
+        h.conn[t] = make_int2(nb, conn_word(nf, p, b, 0));
+        if (!face_scalings(nJ + t * 3, h.nj5.data() + t * 5))
+            return bb_fail(ctx, BB_ERR_ARG, "zero face normal at element %lld", (long long)(t / 4));
+    }
+    for (int64_t e = 0; e < K; ++e)
+        if (!(J[e] > 0.0)) return bb_fail(ctx, BB_ERR_ARG, "non-positive Jacobian at element %lld", (long long)e);
+    h.nJ3.assign(nJ, nJ + K * 12);
+    h.tab = stage_tables(N);
+    h.l0a = l0_mfma_table(N);
+    if (h.l0a.empty())
+        return bb_fail(ctx, BB_ERR_STATE, "L0 has entries outside the band the matrix-core kernel assumes (N=%d)", N);
+    h.invJ.resize(K);
+    h.geoh64.resize(K * 9);
+    h.geoh32.resize(K * 9);
+    for (int64_t e = 0; e < K; ++e) h.invJ[e] = 1.0 / J[e];
+    for (int64_t t = 0; t < K * 9; ++t) {
+        h.geoh64[t] = 0.5 * geo[t];
+        h.geoh32[t] = (float)h.geoh64[t];
+    }
+    return BB_OK;
+}
+
+struct BatchHostGroup {
+    int N = 0;
+    int64_t start = 0, count = 0, qoff = 0;  // first element, elements, state offset (doubles)
+};
+
+struct BatchHost {
+    int F = 4;
+    int64_t words = 0;                      // state length (doubles)
+    std::vector<BatchHostGroup> groups;     // elements grouped by degree
+    std::vector<int2> conn;                 // [K][4]
+    std::vector<int64_t> nboff;             // [K][4] the neighbour's state offset
+    std::vector<double> nj5;                // [K][4][5]
+    std::vector<double> invJ, geoh;         // [K], [K][9] geo / 2
+    std::vector<float> geoh32, trf;         // [K][9]; the transfer tables in FP32
+    // degree elevations (neighbour degree below the element's) as sparse rows:
+    // eoff[11 N + Nn] -> first row of erow; row r's entries are ecol/evd[erow[r] .. erow[r+1])
+    std::vector<int32_t> erow;
+    std::vector<int64_t> eoff;
+    std::vector<uint8_t> ecol;
+    std::vector<double> evd;
+    std::vector<float> evf;
+};
+
+// is the transfer matrix from degree Nn to N present in a table of transfer_len doubles?
+inline bool transfer_present(const int64_t *toff, int64_t transfer_len, int N, int Nn) {
+    const int64_t o = toff[11 * N + Nn];
+    return o >= 0 && o + (int64_t)nfp2(N) * nfp2(Nn) <= transfer_len;
+}
+
+// validates the arguments of bb_batch_create and builds every host array it uploads
+inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_t *deg, const double *geo,
+                            const double *J, const double *nJ, const int32_t *nbr, const int32_t *nbr_face,
+                            const int32_t *perm, const int32_t *bc, const double *dt, const double *transfer,
+                            const int64_t *toff, int64_t transfer_len, BatchHost &h) {
+    if (!ctx) return BB_ERR_NULL;
+    if (pde != BB_PDE_WAVE) return bb_fail(ctx, BB_ERR_UNSUPPORTED, "batches of mixed degree: wave system only");
+    if (flux_kind != BB_FLUX_REFERENCE && flux_kind != BB_FLUX_UPWIND)
+        return bb_fail(ctx, BB_ERR_ARG, "flux %d", flux_kind);
+    if (K < 1) return bb_fail(ctx, BB_ERR_ARG, "K must be >= 1");
+    if (K > kMaxElements) return bb_fail(ctx, BB_ERR_ARG, "element count K=%lld too large", (long long)K);
+    if (transfer_len < 0) return bb_fail(ctx, BB_ERR_ARG, "transfer_len %lld < 0", (long long)transfer_len);
+    if (!deg || !geo || !J || !nJ || !nbr || !nbr_face || !perm || !bc || !dt || !transfer || !toff)
+        return bb_fail(ctx, BB_ERR_NULL, "null batch array");
+    const int F = h.F;
+    std::vector<int64_t> qoff_e(K);
+    int64_t words = 0;
+    for (int64_t e = 0; e < K; ++e) {
+        const int N = deg[e];
+        if (N < 1 || N > kStageMaxN) return bb_fail(ctx, BB_ERR_ARG, "degree %d of element %lld", N, (long long)e);
+        if (e && N < deg[e - 1]) return bb_fail(ctx, BB_ERR_ARG, "elements must be grouped by increasing degree");
+        if (!(J[e] > 0.0)) return bb_fail(ctx, BB_ERR_ARG, "non-positive Jacobian at element %lld", (long long)e);
+        qoff_e[e] = words;
+        words += (int64_t)F * np3(N);
+    }
+    h.words = words;
+    h.conn.resize(K * 4);
+    h.nboff.assign(K * 4, 0);
+    h.nj5.resize(K * 20);
+    for (int64_t t = 0; t < K * 4; ++t) {
+        const int64_t e = t / 4;
+        const int nb = nbr[t], nf = nbr_face[t], p = perm[t], b = bc[t];
+        if (nb < 0 || nb >= K || nf < 0 || nf > 3 || p < 0 || p > 5 || b < 0 || b > 1)
+            return bb_fail(ctx, BB_ERR_ARG, "bad connectivity at element %lld face %d", (long long)e, (int)(t % 4));
+        const int N = deg[e], Nn = deg[nb];
+        if (!b && Nn != N && !transfer_present(toff, transfer_len, N, Nn))
+            return bb_fail(ctx, BB_ERR_ARG, "no transfer table from degree %d to %d", Nn, N);
+        h.conn[t] = make_int2(nb, conn_word(nf, p, b, Nn));
+        h.nboff[t] = b ? 0 : qoff_e[nb];
+        if (!face_scalings(nJ + t * 3, h.nj5.data() + t * 5))
+            return bb_fail(ctx, BB_ERR_ARG, "zero face normal at element %lld", (long long)e);
+    }
+    for (int64_t e = 0; e < K;) {
+        BatchHostGroup g;
+        g.N = deg[e];
+        g.start = e;
+        g.qoff = qoff_e[e];
+        while (e < K && deg[e] == g.N) ++e;
+        g.count = e - g.start;
+        h.groups.push_back(g);
+    }
+    h.eoff.assign(121, -1);
+    for (int N = 1; N <= kStageMaxN; ++N)
+        for (int Nn = 1; Nn < N; ++Nn) {
+            if (!transfer_present(toff, transfer_len, N, Nn)) continue;
+            const int64_t o = toff[11 * N + Nn];
+            h.eoff[11 * N + Nn] = (int64_t)h.erow.size();
+            for (int r = 0; r < nfp2(N); ++r) {
+                h.erow.push_back((int32_t)h.evd.size());
+                for (int c = 0; c < nfp2(Nn); ++c) {
+                    const double v = transfer[o + (int64_t)r * nfp2(Nn) + c];
+                    if (v != 0.0) {
+                        h.ecol.push_back((uint8_t)c);
+                        h.evd.push_back(v);
+                    }
+                }
+            }
+            h.erow.push_back((int32_t)h.evd.size());
+        }
+    h.evf.assign(h.evd.begin(), h.evd.end());
+    h.trf.assign(transfer, transfer + transfer_len);
+    h.invJ.resize(K);
+    h.geoh.resize(K * 9);
+    h.geoh32.resize(K * 9);
+    for (int64_t e = 0; e < K; ++e) h.invJ[e] = 1.0 / J[e];
+    for (int64_t t = 0; t < K * 9; ++t) {
+        h.geoh[t] = 0.5 * geo[t];
+        h.geoh32[t] = (float)h.geoh[t];
+    }
+    return BB_OK;
+}
+
+}  // namespace bb

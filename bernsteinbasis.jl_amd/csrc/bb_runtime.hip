@@ -1,31 +1,9 @@
 // bb_runtime.hip -- context, device memory, timing and the host-side tables of the C ABI.
-#include <stdarg.h>
 #include <string.h>
 
 #include <vector>
 
 #include "bb_common.hpp"
-
-int bb_fail(bb_ctx *ctx, int code, const char *fmt, ...) {
-    if (ctx) {
-        char buf[1024];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof buf, fmt, ap);
-        va_end(ap);
-        ctx->err = buf;
-    }
-    return code;
-}
-
-int bb_check_common(bb_ctx *ctx, int N, int F, int64_t K, int nmin, int nmax) {
-    if (!ctx) return BB_ERR_NULL;
-    if (N < nmin || N > nmax) return bb_fail(ctx, BB_ERR_ARG, "degree N=%d outside [%d,%d]", N, nmin, nmax);
-    if (F < 1 || F > 64) return bb_fail(ctx, BB_ERR_ARG, "field count F=%d outside [1,64]", F);
-    if (K < 0) return bb_fail(ctx, BB_ERR_ARG, "element count K=%lld < 0", (long long)K);
-    if (K > (int64_t(1) << 31) / 4) return bb_fail(ctx, BB_ERR_ARG, "element count K=%lld too large", (long long)K);
-    return BB_OK;
-}
 
 extern "C" {
 
@@ -46,7 +24,8 @@ int bb_ctx_create(int device, bb_ctx **out) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return BB_ERR_HIP;
     if (device < 0 || device >= n) return BB_ERR_ARG;
-    bb_ctx *c = new bb_ctx();
+    bb_ctx *c = new (std::nothrow) bb_ctx();
+    if (!c) return BB_ERR_NOMEM;
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -69,7 +48,7 @@ int bb_ctx_destroy(bb_ctx *ctx) {
     return BB_OK;
 }
 
-const char *bb_last_error(const bb_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *bb_last_error(const bb_ctx *ctx) { return ctx ? ctx->err : "null context"; }
 
 int bb_ctx_stream(bb_ctx *ctx, void **stream) {
     if (!ctx || !stream) return BB_ERR_NULL;

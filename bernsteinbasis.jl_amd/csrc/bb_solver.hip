@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "bb_dispatch.hpp"
+#include "bb_host.hpp"
 #include "bb_stage.hpp"
 
 using namespace bb;
@@ -30,6 +31,7 @@ struct bb_solver {
     int8_t *bcn = nullptr;
     double *nx = nullptr, *ny = nullptr, *nz = nullptr;
     double *uM = nullptr, *flux = nullptr, *fu[3] = {nullptr, nullptr, nullptr}, *d9[9] = {}, *du = nullptr;
+    bool plain_ready = false;  // every plain-path array allocated and filled
     // state
     double *q[2] = {nullptr, nullptr};
     int cur = 0;
@@ -92,7 +94,7 @@ static int launch_stage(bb_solver *s, int prec, int tiling, const StageArgs &a) 
     int rc = stage_fn(s->pde, prec)(s->N, tiling, a, s->ctx->stream, false);
     if (rc == BB_ERR_UNSUPPORTED)
         return bb_fail(s->ctx, rc, "no fused kernel for N=%d with tiling %d", s->N, tiling);
-    if (rc) return bb_fail(s->ctx, rc, "fused stage launch failed: %s", hipGetErrorString(hipGetLastError()));
+    if (rc) return bb_fail(s->ctx, rc, "fused stage launch failed: %s", hipGetErrorString(t_launch_err));
     return BB_OK;
 }
 
@@ -112,8 +114,30 @@ static StageArgs stage_args(bb_solver *s, int prec) {
 }
 
 // ---- plain path: one kernel per operator of the reference RHS
+// frees the plain path's arrays (after a failed set-up, before it is tried again)
+static void free_plain(bb_solver *s) {
+    void **ptrs[] = {(void **)&s->mapP, (void **)&s->bcn, (void **)&s->nx, (void **)&s->ny, (void **)&s->nz,
+                     (void **)&s->uM, (void **)&s->flux, (void **)&s->du};
+    for (void **p : ptrs)
+        if (*p) {
+            hipFree(*p);
+            *p = nullptr;
+        }
+    for (auto &p : s->fu)
+        if (p) {
+            hipFree(p);
+            p = nullptr;
+        }
+    for (auto &p : s->d9)
+        if (p) {
+            hipFree(p);
+            p = nullptr;
+        }
+}
+
 static int ensure_plain(bb_solver *s) {
-    if (s->mapP) return BB_OK;
+    if (s->plain_ready) return BB_OK;
+    free_plain(s);
     bb_ctx *ctx = s->ctx;
     const int NFP = s->NFP, N = s->N;
     const int64_t M = s->K * 4 * NFP;
@@ -162,6 +186,7 @@ static int ensure_plain(bb_solver *s) {
     for (auto &p : s->fu) BB_HIP(ctx, hipMalloc(&p, vw * sizeof(double)));
     for (auto &p : s->d9) BB_HIP(ctx, hipMalloc(&p, vw * sizeof(double)));
     BB_HIP(ctx, hipMalloc(&s->du, vw * sizeof(double)));
+    s->plain_ready = true;
     return BB_OK;
 }
 
@@ -233,42 +258,26 @@ static void drop_graphs(bb_solver *s) {
 
 extern "C" {
 
-int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *geo_host,
-                     const double *J_host, const double *nJ_host, const int32_t *nbr_host,
-                     const int8_t *nbr_face_host, const int8_t *perm_host, const int8_t *bc_host,
-                     const double *bc_sign_host, bb_solver **out) {
-    if (!ctx || !out) return BB_ERR_NULL;
-    *out = nullptr;
-    if (pde != BB_PDE_WAVE && pde != BB_PDE_MAXWELL) return bb_fail(ctx, BB_ERR_ARG, "pde %d", pde);
-    if (flux_kind != BB_FLUX_REFERENCE && flux_kind != BB_FLUX_UPWIND) return bb_fail(ctx, BB_ERR_ARG, "flux %d", flux_kind);
-    const int F = pde == BB_PDE_WAVE ? 4 : 6;
-    int rc = bb_check_common(ctx, N, F, K, 1, kStageMaxN);
+static int solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *geo_host,
+                         const double *J_host, const double *nJ_host, const int32_t *nbr_host,
+                         const int8_t *nbr_face_host, const int8_t *perm_host, const int8_t *bc_host,
+                         const double *bc_sign_host, bb_solver **out) {
+    SolverHost h;
+    int rc = solver_host_setup(ctx, pde, flux_kind, N, K, geo_host, J_host, nJ_host, nbr_host, nbr_face_host,
+                               perm_host, bc_host, h);
     if (rc) return rc;
-    if (K < 1) return bb_fail(ctx, BB_ERR_ARG, "K must be >= 1");
-    if (!geo_host || !J_host || !nJ_host || !nbr_host || !nbr_face_host || !perm_host || !bc_host)
-        return bb_fail(ctx, BB_ERR_NULL, "null mesh array");
-    std::vector<int2> conn(K * 4);
-    for (int64_t t = 0; t < K * 4; ++t) {
-        const int nb = nbr_host[t], nf = nbr_face_host[t], p = perm_host[t], bc = bc_host[t];
-        if (nb < 0 || nb >= K || nf < 0 || nf > 3 || p < 0 || p > 5 || bc < 0 || bc > 1)
-            return bb_fail(ctx, BB_ERR_ARG, "bad connectivity at element %lld face %d", (long long)(t / 4), (int)(t % 4));
-        conn[t] = make_int2(nb, nf | (p << 2) | (bc << 5));
-    }
-    for (int64_t e = 0; e < K; ++e)
-        if (!(J_host[e] > 0.0)) return bb_fail(ctx, BB_ERR_ARG, "non-positive Jacobian at element %lld", (long long)e);
-    bb_solver *s = new bb_solver();
+    bb_solver *s = new (std::nothrow) bb_solver();
+    if (!s) return bb_fail(ctx, BB_ERR_NOMEM, "host memory allocation failed");
     s->ctx = ctx;
     s->pde = pde;
-    s->F = F;
+    s->F = h.F;
     s->N = N;
     s->K = K;
     s->flux_kind = flux_kind;
-    s->NP = np3(N);
-    s->NFP = nfp2(N);
+    s->NP = h.NP;
+    s->NFP = h.NFP;
     if (bc_sign_host)
-        for (int f = 0; f < F; ++f) s->sign[f] = bc_sign_host[f];
-    s->conn_host = conn;
-    s->nJ_host.assign(nJ_host, nJ_host + K * 12);
+        for (int f = 0; f < h.F; ++f) s->sign[f] = bc_sign_host[f];
     auto fail = [&](hipError_t e) {
         int r = bb_fail(ctx, BB_ERR_HIP, "solver allocation: %s", hipGetErrorString(e));
         bb_solver_destroy(s);
@@ -276,56 +285,31 @@ int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, cons
     };
     hipError_t e;
     const size_t vw = state_words(s);
-    std::vector<double> nj5(K * 20);
-    for (int64_t t = 0; t < K * 4; ++t) {
-        const double *n = nJ_host + t * 3;
-        const double Jf = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-        if (!(Jf > 0.0)) {
-            delete s;
-            return bb_fail(ctx, BB_ERR_ARG, "zero face normal at element %lld", (long long)(t / 4));
-        }
-        double *o = nj5.data() + t * 5;
-        o[0] = n[0];
-        o[1] = n[1];
-        o[2] = n[2];
-        o[3] = Jf;
-        o[4] = 1.0 / Jf;
-    }
-    const std::vector<uint16_t> tab = stage_tables(N);
-    const std::vector<float> l0a = l0_mfma_table(N);
-    if (l0a.empty()) {
-        delete s;
-        return bb_fail(ctx, BB_ERR_STATE, "L0 has entries outside the band the matrix-core kernel assumes (N=%d)", N);
-    }
-    std::vector<double> invJ(K), geoh64(K * 9);
-    std::vector<float> geoh32(K * 9);
-    for (int64_t e2 = 0; e2 < K; ++e2) invJ[e2] = 1.0 / J_host[e2];
-    for (int64_t t = 0; t < K * 9; ++t) {
-        geoh64[t] = 0.5 * geo_host[t];
-        geoh32[t] = (float)geoh64[t];
-    }
     if ((e = hipMalloc(&s->geo, K * 9 * sizeof(double))) || (e = hipMalloc(&s->J, K * sizeof(double))) ||
         (e = hipMalloc(&s->invJ, K * sizeof(double))) || (e = hipMalloc(&s->geoh64, K * 9 * sizeof(double))) ||
         (e = hipMalloc(&s->geoh32, K * 9 * sizeof(float))) ||
         (e = hipMalloc(&s->nJ, K * 20 * sizeof(double))) || (e = hipMalloc(&s->conn, K * 4 * sizeof(int2))) ||
-        (e = hipMalloc(&s->tab, tab.size() * sizeof(uint16_t))) ||
-        (e = hipMalloc(&s->l0a, l0a.size() * sizeof(float))) ||
+        (e = hipMalloc(&s->tab, h.tab.size() * sizeof(uint16_t))) ||
+        (e = hipMalloc(&s->l0a, h.l0a.size() * sizeof(float))) ||
         (e = hipMalloc(&s->q[0], vw * sizeof(double))) || (e = hipMalloc(&s->q[1], vw * sizeof(double))))
         return fail(e);
     if ((e = hipMemcpy(s->geo, geo_host, K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(s->J, J_host, K * sizeof(double), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->invJ, invJ.data(), K * sizeof(double), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->geoh64, geoh64.data(), K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->geoh32, geoh32.data(), K * 9 * sizeof(float), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->nJ, nj5.data(), K * 20 * sizeof(double), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->conn, conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->tab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(s->l0a, l0a.data(), l0a.size() * sizeof(float), hipMemcpyHostToDevice)))
+        (e = hipMemcpy(s->invJ, h.invJ.data(), K * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->geoh64, h.geoh64.data(), K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->geoh32, h.geoh32.data(), K * 9 * sizeof(float), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->nJ, h.nj5.data(), K * 20 * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->conn, h.conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->tab, h.tab.data(), h.tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(s->l0a, h.l0a.data(), h.l0a.size() * sizeof(float), hipMemcpyHostToDevice)))
         return fail(e);
     if ((rc = bb_fill_zero(ctx, s->q[0], vw * sizeof(double))) || (rc = bb_fill_zero(ctx, s->q[1], vw * sizeof(double)))) {
         bb_solver_destroy(s);
         return rc;
     }
+    // kept for the plain path's face-node arrays (built on its first use); swaps do not allocate
+    s->conn_host.swap(h.conn);
+    s->nJ_host.swap(h.nJ3);
     *out = s;
     return BB_OK;
 }
@@ -368,7 +352,7 @@ int bb_solver_state_ptr(bb_solver *s, double **q_dev) {
     return BB_OK;
 }
 
-int bb_solver_rhs(bb_solver *s, int path, int prec, double *du_host) {
+static int solver_rhs(bb_solver *s, int path, int prec, double *du_host) {
     if (!s || !du_host) return BB_ERR_NULL;
     int rc = check_path(s, path, prec);
     if (rc) return rc;
@@ -399,7 +383,7 @@ int bb_solver_rhs(bb_solver *s, int path, int prec, double *du_host) {
     return BB_OK;
 }
 
-int bb_solver_step(bb_solver *s, int path, int prec, double dt, int nsteps) {
+static int solver_step(bb_solver *s, int path, int prec, double dt, int nsteps) {
     if (!s) return BB_ERR_NULL;
     if (nsteps < 0) return bb_fail(s->ctx, BB_ERR_ARG, "nsteps %d", nsteps);
     int rc = check_path(s, path, prec);
@@ -407,7 +391,7 @@ int bb_solver_step(bb_solver *s, int path, int prec, double dt, int nsteps) {
     return enqueue_steps(s, path, prec, dt, nsteps, &s->cur);
 }
 
-int bb_solver_graph_capture(bb_solver *s, int path, int prec, double dt, int nsteps) {
+static int solver_graph_capture(bb_solver *s, int path, int prec, double dt, int nsteps) {
     if (!s) return BB_ERR_NULL;
     if (nsteps < 1) return bb_fail(s->ctx, BB_ERR_ARG, "nsteps %d", nsteps);
     int rc = check_path(s, path, prec);
@@ -458,7 +442,7 @@ int bb_solver_set_tuning(bb_solver *s, int prec, int tiling) {
     return BB_OK;
 }
 
-int bb_solver_autotune(bb_solver *s, int prec, int *tiling) {
+static int solver_autotune(bb_solver *s, int prec, int *tiling) {
     if (!s) return BB_ERR_NULL;
     int rc = check_path(s, BB_PATH_FUSED, prec);
     if (rc) return rc;
@@ -504,6 +488,40 @@ int bb_solver_stage_bytes(bb_solver *s, int prec, double *bytes, double *dofs) {
     if (bytes) *bytes = w * (8.0 + 8.0 + 2.0 * sres) + (double)s->K * (9 * 8 + 8 + 20 * 8 + 4 * 8);
     if (dofs) *dofs = w;
     return BB_OK;
+}
+
+// Entry points that allocate on the host (set-up tables, the plain path's face-node arrays,
+// widened copies) run under bb_guard: an exception becomes BB_ERR_NOMEM.
+int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *geo_host,
+                     const double *J_host, const double *nJ_host, const int32_t *nbr_host,
+                     const int8_t *nbr_face_host, const int8_t *perm_host, const int8_t *bc_host,
+                     const double *bc_sign_host, bb_solver **out) {
+    if (!ctx || !out) return BB_ERR_NULL;
+    *out = nullptr;
+    return bb_guard(ctx, [&] {
+        return solver_create(ctx, pde, flux_kind, N, K, geo_host, J_host, nJ_host, nbr_host, nbr_face_host,
+                             perm_host, bc_host, bc_sign_host, out);
+    });
+}
+
+int bb_solver_rhs(bb_solver *s, int path, int prec, double *du_host) {
+    if (!s || !du_host) return BB_ERR_NULL;
+    return bb_guard(s->ctx, [&] { return solver_rhs(s, path, prec, du_host); });
+}
+
+int bb_solver_step(bb_solver *s, int path, int prec, double dt, int nsteps) {
+    if (!s) return BB_ERR_NULL;
+    return bb_guard(s->ctx, [&] { return solver_step(s, path, prec, dt, nsteps); });
+}
+
+int bb_solver_graph_capture(bb_solver *s, int path, int prec, double dt, int nsteps) {
+    if (!s) return BB_ERR_NULL;
+    return bb_guard(s->ctx, [&] { return solver_graph_capture(s, path, prec, dt, nsteps); });
+}
+
+int bb_solver_autotune(bb_solver *s, int prec, int *tiling) {
+    if (!s) return BB_ERR_NULL;
+    return bb_guard(s->ctx, [&] { return solver_autotune(s, prec, tiling); });
 }
 
 }  // extern "C"

@@ -953,7 +953,7 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
                             rc = BB_OK;                                                                \
                         } else {                                                                       \
                             k_stage<NN, PDE, MIXED, E, LMV><<<grid1d(K, E), Cfg::NT, 0, st>>>(a);      \
-                            rc = hipGetLastError() == hipSuccess ? BB_OK : BB_ERR_HIP;                 \
+                            rc = launch_status();                                                      \
                         }                                                                              \
                     }                                                                                  \
                 }                                                                                      \
@@ -999,7 +999,7 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
                             rc = BB_OK;                                                                \
                         } else {                                                                       \
                             k_stage<NN, PDE, MIXED, E, LMV, true><<<grid1d(K, E), Cfg::NT, 0, st>>>(a); \
-                            rc = hipGetLastError() == hipSuccess ? BB_OK : BB_ERR_HIP;                 \
+                            rc = launch_status();                                                      \
                         }                                                                              \
                     }                                                                                  \
                 }                                                                                      \

@@ -36,7 +36,8 @@ enum bb_status {
     BB_ERR_NULL = 2,        /* null pointer */
     BB_ERR_HIP = 3,         /* HIP runtime error (message in bb_last_error) */
     BB_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
-    BB_ERR_STATE = 5        /* call out of order (e.g. graph launch before capture) */
+    BB_ERR_STATE = 5,       /* call out of order (e.g. graph launch before capture) */
+    BB_ERR_NOMEM = 6        /* host memory allocation (or other host-side) failure */
 };
 
 enum bb_dtype { BB_F64 = 0, BB_F32 = 1 };

@@ -74,12 +74,21 @@ def baseline_lib():
     return _baseline
 
 
+def build_sanitized(out: str) -> str:
+    """the checker built with the address and undefined-behaviour sanitizers (host tests)"""
+    return _compile(out, CHECK_FLAGS + ["-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                                        "-fno-omit-frame-pointer"])
+
+
 def lib():
+    """the checker build; BB_ORACLE_SO names another build of bbref.c to load instead (the
+    sanitizer run of the oracle tests)"""
     global _lib
     if _lib is None:
-        if not os.path.exists(SO):
+        path = os.environ.get("BB_ORACLE_SO") or SO
+        if path == SO and not os.path.exists(SO):
             build()
-        _lib = _bind(C.CDLL(SO))
+        _lib = _bind(C.CDLL(path))
     return _lib
 
 

@@ -1,0 +1,187 @@
+// Host-side set-up of libbbdg under the address and undefined-behaviour sanitizers, with
+// no GPU: the stage kernel's index tables and matrix-core L0 tables of every degree, the
+// solver and batch set-up on small valid inputs and on each input they must reject, the
+// exception guard of the entry points, and the fixed-size error buffer.  Built host-only
+// (no device code) by tests/test_sanitizers.py; prints "host checks passed" and exits 0.
+#include <stdexcept>
+#include <string>
+
+#include "bb_host.hpp"
+
+using namespace bb;
+
+static int g_fail = 0, g_checks = 0;
+#define CHECK(cond)                                                            \
+    do {                                                                       \
+        ++g_checks;                                                            \
+        if (!(cond)) {                                                         \
+            ++g_fail;                                                          \
+            fprintf(stderr, "check failed at line %d: %s\n", __LINE__, #cond); \
+        }                                                                      \
+    } while (0)
+
+static void check_tables() {
+    for (int N = 1; N <= kStageMaxN; ++N) {
+        const std::vector<uint16_t> t = stage_tables(N);
+        const int NP = np3(N), NFP = nfp2(N);
+        CHECK(t.size() % 8 == 0 && (int)t.size() >= 2 * NP + 28 * NFP + 4 * NP);
+        for (int n = 0; n < NP; ++n) {  // node -> (i, j, k) decodes back to n
+            const int i = t[n] & 15, j = (t[n] >> 4) & 15, k = t[n] >> 8;
+            CHECK(lin3(N, i, j, k) == n);
+        }
+        for (int f = 0; f < 4; ++f)  // own face nodes
+            for (int m = 0; m < NFP; ++m) {
+                const int ab = t[NP + np3(N - 1) + m];
+                CHECK(t[2 * NP + f * NFP + m] == face_vidx(N, f, ab & 255, ab >> 8));
+            }
+        const std::vector<float> l0 = l0_mfma_table(N);
+        CHECK(!l0.empty());
+        double sum = 0;
+        for (float v : l0) {
+            CHECK(v == (float)(int)v && v >= 0);  // integer entries of 2 L0, exact in FP32
+            sum += v;
+        }
+        // 2 L0 = (N+1)^2 E^T E with E the elevation N -> N+1, whose rows sum to 1, so the
+        // entries of E^T E sum to the number of rows of E, nfp(N+1)
+        CHECK(sum == (double)(N + 1) * (N + 1) * nfp2(N + 1));
+    }
+}
+
+static void check_solver_setup() {
+    bb_ctx ctx;
+    const int N = 3;
+    const int64_t K = 1;
+    double geo[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, J[1] = {0.5};
+    double nJ[12] = {0, -1, 0, 1, 1, 1, -1, 0, 0, 0, 0, -1};
+    int32_t nbr[4] = {0, 0, 0, 0};
+    int8_t nf[4] = {0, 1, 2, 3}, perm[4] = {0, 0, 0, 0}, bc[4] = {1, 1, 1, 1};
+    auto run = [&](int pde, int flux, int n, int64_t k, const double *g, const int32_t *nb, const int8_t *f,
+                   const int8_t *p, const int8_t *b, const double *j, const double *nj) {
+        SolverHost h;
+        return solver_host_setup(&ctx, pde, flux, n, k, g, j, nj, nb, f, p, b, h);
+    };
+    {
+        SolverHost h;
+        CHECK(solver_host_setup(&ctx, BB_PDE_MAXWELL, BB_FLUX_UPWIND, N, K, geo, J, nJ, nbr, nf, perm, bc, h) == BB_OK);
+        CHECK(h.F == 6 && h.NP == 20 && h.NFP == 10 && h.conn.size() == 4 && h.nj5.size() == 20);
+        CHECK(h.conn[2].x == 0 && h.conn[2].y == (2 | (1 << 5)));
+        CHECK(h.nj5[5 + 3] == sqrt(3.0) && h.nj5[4] == 1.0 && h.invJ[0] == 2.0 && h.geoh64[0] == 0.5);
+        CHECK(h.tab == stage_tables(N) && !h.l0a.empty());
+    }
+    {
+        SolverHost h;
+        CHECK(solver_host_setup(nullptr, 0, 1, N, K, geo, J, nJ, nbr, nf, perm, bc, h) == BB_ERR_NULL);
+    }
+    CHECK(run(2, 1, N, K, geo, nbr, nf, perm, bc, J, nJ) == BB_ERR_ARG);
+    CHECK(run(0, 2, N, K, geo, nbr, nf, perm, bc, J, nJ) == BB_ERR_ARG);
+    CHECK(run(0, 1, 0, K, geo, nbr, nf, perm, bc, J, nJ) == BB_ERR_ARG);
+    CHECK(run(0, 1, kStageMaxN + 1, K, geo, nbr, nf, perm, bc, J, nJ) == BB_ERR_ARG);
+    CHECK(run(0, 1, N, 0, geo, nbr, nf, perm, bc, J, nJ) == BB_ERR_ARG);
+    CHECK(run(0, 1, N, -1, geo, nbr, nf, perm, bc, J, nJ) == BB_ERR_ARG);
+    CHECK(run(0, 1, N, kMaxElements + 1, geo, nbr, nf, perm, bc, J, nJ) == BB_ERR_ARG);
+    CHECK(run(0, 1, N, K, nullptr, nbr, nf, perm, bc, J, nJ) == BB_ERR_NULL);
+    CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bc, nullptr, nJ) == BB_ERR_NULL);
+    {
+        int32_t bad_nbr[4] = {0, 1, 0, 0};  // neighbour 1 of a one-element mesh
+        CHECK(run(0, 1, N, K, geo, bad_nbr, nf, perm, bc, J, nJ) == BB_ERR_ARG);
+        int8_t bad_nf[4] = {0, 4, 2, 3}, bad_perm[4] = {0, 0, 6, 0}, bad_bc[4] = {1, 2, 1, 1};
+        CHECK(run(0, 1, N, K, geo, nbr, bad_nf, perm, bc, J, nJ) == BB_ERR_ARG);
+        CHECK(run(0, 1, N, K, geo, nbr, nf, bad_perm, bc, J, nJ) == BB_ERR_ARG);
+        CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bad_bc, J, nJ) == BB_ERR_ARG);
+        double J0[1] = {0.0}, Jnan[1] = {NAN}, nJ0[12] = {0, -1, 0, 0, 0, 0, -1, 0, 0, 0, 0, -1};
+        CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bc, J0, nJ) == BB_ERR_ARG);
+        CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bc, Jnan, nJ) == BB_ERR_ARG);
+        CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bc, J, nJ0) == BB_ERR_ARG);
+        CHECK(std::string(ctx.err).find("zero face normal") != std::string::npos);
+    }
+}
+
+static void check_batch_setup() {
+    bb_ctx ctx;
+    // two elements of degrees 3 and 4 sharing face 0, all other faces walls
+    const int64_t K = 2;
+    int32_t deg[2] = {3, 4};
+    double geo[18], J[2] = {1, 1}, dt[2] = {1e-3, 1e-3}, nJ[24];
+    for (int t = 0; t < 18; ++t) geo[t] = (t % 4 == 0) ? 1.0 : 0.0;
+    for (int t = 0; t < 8; ++t) {
+        nJ[3 * t] = 0;
+        nJ[3 * t + 1] = -1;
+        nJ[3 * t + 2] = 0;
+    }
+    int32_t nbr[8] = {1, 0, 0, 0, 0, 1, 1, 1}, nf[8] = {0, 1, 2, 3, 0, 1, 2, 3}, perm[8] = {0};
+    int32_t bc[8] = {0, 1, 1, 1, 0, 1, 1, 1};
+    // transfer tables 3 <- 4 (projection, dense) and 4 <- 3 (elevation, sparse)
+    const int n3 = nfp2(3), n4 = nfp2(4);
+    std::vector<double> tr((size_t)2 * n3 * n4, 0.0);
+    std::vector<int64_t> toff(121, -1);
+    toff[11 * 3 + 4] = 0;
+    toff[11 * 4 + 3] = n3 * n4;
+    for (int r = 0; r < n3; ++r)
+        for (int c = 0; c < n4; ++c) tr[(size_t)r * n4 + c] = 1.0 / n4;
+    for (int r = 0; r < n4; ++r) tr[(size_t)n3 * n4 + (size_t)r * n3 + r % n3] = 1.0;  // one entry per row
+    const int64_t tl = (int64_t)tr.size();
+    auto run = [&](int pde, int64_t k, const int32_t *d, const int64_t *to, int64_t len) {
+        BatchHost h;
+        return batch_host_setup(&ctx, pde, BB_FLUX_UPWIND, k, d, geo, J, nJ, nbr, nf, perm, bc, dt, tr.data(), to, len,
+                                h);
+    };
+    {
+        BatchHost h;
+        CHECK(batch_host_setup(&ctx, BB_PDE_WAVE, BB_FLUX_UPWIND, K, deg, geo, J, nJ, nbr, nf, perm, bc, dt, tr.data(),
+                               toff.data(), tl, h) == BB_OK);
+        CHECK(h.groups.size() == 2 && h.groups[1].qoff == 4 * np3(3) && h.words == 4 * (np3(3) + np3(4)));
+        CHECK(h.nboff[0] == 4 * np3(3) && h.nboff[4] == 0 && (h.conn[0].y >> 8) == 4 && (h.conn[4].y >> 8) == 3);
+        CHECK(h.eoff[11 * 4 + 3] == 0 && h.erow.size() == (size_t)n4 + 1 && h.evd.size() == (size_t)n4);
+        CHECK(h.trf.size() == tr.size() && h.evf.size() == h.evd.size());
+    }
+    CHECK(run(BB_PDE_MAXWELL, K, deg, toff.data(), tl) == BB_ERR_UNSUPPORTED);
+    CHECK(run(BB_PDE_WAVE, 0, deg, toff.data(), tl) == BB_ERR_ARG);
+    CHECK(run(BB_PDE_WAVE, kMaxElements + 1, deg, toff.data(), tl) == BB_ERR_ARG);
+    CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), -1) == BB_ERR_ARG);
+    CHECK(std::string(ctx.err).find("transfer_len") != std::string::npos);
+    CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), tl - 1) == BB_ERR_ARG);  // the 4 <- 3 table is cut short
+    CHECK(run(BB_PDE_WAVE, K, deg, nullptr, tl) == BB_ERR_NULL);
+    int32_t down[2] = {4, 3}, high[2] = {3, kStageMaxN + 1};
+    CHECK(run(BB_PDE_WAVE, K, down, toff.data(), tl) == BB_ERR_ARG);
+    CHECK(run(BB_PDE_WAVE, K, high, toff.data(), tl) == BB_ERR_ARG);
+    std::vector<int64_t> missing = toff;
+    missing[11 * 3 + 4] = -1;
+    CHECK(run(BB_PDE_WAVE, K, deg, missing.data(), tl) == BB_ERR_ARG);
+    CHECK(std::string(ctx.err).find("no transfer table from degree 4 to 3") != std::string::npos);
+}
+
+static void check_guard_and_errors() {
+    bb_ctx ctx;
+    CHECK(bb_guard(&ctx, [] { return 7; }) == 7);
+    CHECK(bb_guard(&ctx, []() -> int { throw std::bad_alloc(); }) == BB_ERR_NOMEM);
+    CHECK(std::string(ctx.err) == "host memory allocation failed");
+    CHECK(bb_guard(&ctx, []() -> int { throw std::runtime_error("boom"); }) == BB_ERR_NOMEM);
+    CHECK(std::string(ctx.err).find("boom") != std::string::npos);
+    CHECK(bb_guard(&ctx, []() -> int { throw 3; }) == BB_ERR_NOMEM);
+    CHECK(bb_guard(nullptr, []() -> int { throw std::bad_alloc(); }) == BB_ERR_NOMEM);
+    // an exception thrown by the standard library inside a guarded body (a request beyond
+    // max_size() throws std::length_error before anything is allocated)
+    CHECK(bb_guard(&ctx, [] {
+              std::vector<double> v;
+              v.reserve(v.max_size() + 1);
+              return 0;
+          }) == BB_ERR_NOMEM);
+    std::string big(4000, 'x');
+    CHECK(bb_fail(&ctx, BB_ERR_ARG, "%s", big.c_str()) == BB_ERR_ARG);
+    CHECK(std::string(ctx.err).size() == sizeof ctx.err - 1);
+    CHECK(bb_check_common(nullptr, 3, 4, 1, 1, 20) == BB_ERR_NULL);
+    CHECK(bb_check_common(&ctx, 3, 65, 1, 1, 20) == BB_ERR_ARG);
+}
+
+int main() {
+    check_tables();
+    check_solver_setup();
+    check_batch_setup();
+    check_guard_and_errors();
+    if (g_fail) {
+        fprintf(stderr, "%d of %d host checks failed\n", g_fail, g_checks);
+        return 1;
+    }
+    printf("host checks passed: %d\n", g_checks);
+    return 0;
+}

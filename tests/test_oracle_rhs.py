@@ -124,6 +124,42 @@ def test_upwind_energy_decays(orc):
     assert e[-1] < e[0]
 
 
+@pytest.mark.parametrize("pde,F", [(0, 4), (1, 6)])
+def test_flux_kinds_agree_on_unit_normals_and_scale_with_jf(orc, pde, F):
+    """The reference's flux (kind 0, examples/3d_wave.jl:45-54, examples/3d_maxwell.jl:45-63)
+    and the Jf-consistent one (kind 1) are the same upwind flux written for different
+    normal scalings: with nJ = Jf n (n a unit normal), every term of kind 1 carries exactly
+    one factor Jf, so flux_1(d, Jf n) = Jf flux_0(d, n), and the two coincide where
+    |nJ| = 1.  A dropped, mis-signed or mis-scaled penalty term in either breaks this."""
+    from types import SimpleNamespace
+
+    N = 2
+    nf = 6
+    K = 40
+    rng = np.random.default_rng(7 + pde)
+    n = rng.normal(size=(K, 4, 3))
+    n /= np.linalg.norm(n, axis=2, keepdims=True)
+    Jf = rng.uniform(0.3, 3.0, size=(K, 4, 1))
+    uM = np.ascontiguousarray(rng.uniform(-1, 1, (K, 4 * nf, F)))
+    # every face node's neighbour value is another random face node (interior faces)
+    mapP = np.ascontiguousarray(rng.permutation(K * 4 * nf).reshape(K, 4 * nf).astype(np.int32))
+
+    def arrays(nrm):
+        rep = [np.ascontiguousarray(np.repeat(nrm[:, :, d], nf, axis=1)) for d in range(3)]
+        return SimpleNamespace(mapP=mapP, bc=np.zeros((K, 4 * nf), np.int8), bc_sign=np.ones(F), nx=rep[0],
+                               ny=rep[1], nz=rep[2])
+
+    f0_unit = orc.flux(pde, N, uM, arrays(n), flux_kind=0)
+    f1_unit = orc.flux(pde, N, uM, arrays(n), flux_kind=1)
+    np.testing.assert_allclose(f1_unit, f0_unit, rtol=0, atol=1e-15 * np.abs(f0_unit).max())
+    f1_scaled = orc.flux(pde, N, uM, arrays(n * Jf), flux_kind=1)
+    jf_nodes = np.repeat(Jf[:, :, 0], nf, axis=1)[:, :, None]
+    np.testing.assert_allclose(f1_scaled, jf_nodes * f0_unit, rtol=1e-14, atol=1e-15)
+    # and the reference flux itself does not scale this way (its penalties carry Jf^0 and Jf^2)
+    f0_scaled = orc.flux(pde, N, uM, arrays(n * Jf), flux_kind=0)
+    assert np.abs(f0_scaled - jf_nodes * f0_unit).max() > 1e-3
+
+
 def test_reference_flux_penalty_is_too_stiff_for_lsrk(orc):
     """The reference's flux (penalties with Jf^0 and Jf^2) blows up under LSRK45 with
     dt = 0.5 h / N^2 on the 8^3 x 6 mesh; the Jf-consistent upwind flux does not."""
