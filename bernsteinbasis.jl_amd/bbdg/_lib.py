@@ -79,6 +79,7 @@ SIGS = {
     "bb_solver_autotune": (i32, [vp, i32, C.POINTER(C.c_int)]),
     "bb_solver_set_tuning": (i32, [vp, i32, i32]),
     "bb_solver_stage_bytes": (i32, [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bb_solver_step_noalloc": (i32, [vp, i32, vp, dbl, i32]),
     "bb_gemm": (i32, [vp, i32, i64, i64, i64, dbl, vp, i64, vp, i64, dbl, vp, i64]),
     "bb_pde_flux_dirs": (i32, [vp, i32, i32, i64, vp, vp, vp, vp]),
     "bb_rhs_combine": (i32, [vp, i32, i32, i64, vp, vp, vp, vp]),

@@ -74,6 +74,10 @@ struct bb_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool capturing = false;
     void *blas = nullptr;  // library GEMM handle, created on first use (bb_gemm)
+    // this device's copy of the library's static scratch array (bb_runtime.hip): its device
+    // address, taken once per device at context creation, and its size in bytes
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
     char err[1024] = {0};  // message of the last failure (fixed size: recording it never allocates)
 };
 

@@ -3,7 +3,35 @@
 
 #include <vector>
 
+#include <mutex>
+
 #include "bb_common.hpp"
+
+// The scratch of the entry points that allocate nothing (bb_solver_step_noalloc): a fixed
+// array in the library's code object, one copy per GPU, held until the process ends.  In
+// host code `scratch` names a host-side stand-in, never the device memory: its device
+// address comes from hipGetSymbolAddress, once per device (with that device current), and
+// only that address is passed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[BB_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// device address of this device's scratch (the current device must be `device`)
+hipError_t scratch_address(int device, void **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[device] = p;
+    }
+    *out = g_scratch_dev[device];
+    return hipSuccess;
+}
+}  // namespace
 
 extern "C" {
 
@@ -23,15 +51,20 @@ int bb_ctx_create(int device, bb_ctx **out) {
     *out = nullptr;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return BB_ERR_HIP;
-    if (device < 0 || device >= n) return BB_ERR_ARG;
+    if (device < 0 || device >= n || device >= kMaxDevices) return BB_ERR_ARG;
     bb_ctx *c = new (std::nothrow) bb_ctx();
     if (!c) return BB_ERR_NOMEM;
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        scratch_address(device, &c->scratch) != hipSuccess) {
+        if (c->ev1) hipEventDestroy(c->ev1);
+        if (c->ev0) hipEventDestroy(c->ev0);
+        if (c->stream) hipStreamDestroy(c->stream);
         delete c;
         return BB_ERR_HIP;
     }
+    c->scratch_bytes = BB_SCRATCH_BYTES;
     *out = c;
     return BB_OK;
 }

@@ -490,6 +490,45 @@ int bb_solver_stage_bytes(bb_solver *s, int prec, double *bytes, double *dofs) {
     return BB_OK;
 }
 
+// The fused steps of bb_solver_step on the caller's state q, with the second state buffer
+// and the residual in the context's scratch array (the device address of this device's
+// copy of the library's static scratch, taken at context creation).  Calls nothing that
+// allocates: the launchers, bb_fail (fixed buffer) and one device-to-device copy.
+int bb_solver_step_noalloc(bb_solver *s, int prec, double *q, double dt, int nsteps) {
+    if (!s || !q) return BB_ERR_NULL;
+    bb_ctx *ctx = s->ctx;
+    if (prec != BB_PREC_F64 && prec != BB_PREC_MIXED) return bb_fail(ctx, BB_ERR_ARG, "prec %d", prec);
+    if (nsteps < 0) return bb_fail(ctx, BB_ERR_ARG, "nsteps %d", nsteps);
+    if (!ctx->scratch) return bb_fail(ctx, BB_ERR_STATE, "no scratch array on device %d", ctx->device);
+    const size_t vw = state_words(s);
+    const size_t qbytes = (vw * sizeof(double) + 255) / 256 * 256;
+    const size_t need = qbytes + vw * (prec == BB_PREC_MIXED ? sizeof(float) : sizeof(double));
+    if (need > ctx->scratch_bytes)
+        return bb_fail(ctx, BB_ERR_ARG, "K*F*Np = %zu needs %zu bytes of scratch; the library holds %zu per GPU", vw,
+                       need, ctx->scratch_bytes);
+    if (!s->tiling[prec]) s->tiling[prec] = default_tiling(s, prec);
+    if (!s->tiling[prec]) return bb_fail(ctx, BB_ERR_UNSUPPORTED, "no fused kernel fits N=%d", s->N);
+    double *buf[2] = {q, (double *)ctx->scratch};
+    void *res = (unsigned char *)ctx->scratch + qbytes;
+    int cur = 0, rc;
+    for (int st = 0; st < nsteps; ++st)
+        for (int stage = 0; stage < 5; ++stage) {
+            StageArgs a = stage_args(s, prec);
+            a.res = res;
+            a.q_in = buf[cur];
+            a.q_out = buf[cur ^ 1];
+            a.ca = kLsrkA[stage];
+            a.cb = kLsrkB[stage];
+            a.dt = dt;
+            a.first = stage == 0;
+            if ((rc = launch_stage(s, prec, s->tiling[prec], a))) return rc;
+            cur ^= 1;
+        }
+    // an odd number of stages leaves the state in the scratch buffer
+    if (cur) BB_HIP(ctx, hipMemcpyAsync(q, buf[1], vw * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return BB_OK;
+}
+
 // Entry points that allocate on the host (set-up tables, the plain path's face-node arrays,
 // widened copies) run under bb_guard: an exception becomes BB_ERR_NOMEM.
 int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *geo_host,

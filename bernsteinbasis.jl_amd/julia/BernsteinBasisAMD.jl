@@ -440,6 +440,14 @@ function stage_bytes(s::BBSolver; prec::Symbol = :mixed)
     (bytes = b[], dofs = d[])
 end
 
+"""`step_noalloc!(s, q, dt, nsteps; prec)`: the fused steps of `step!` on the caller's device
+state q (Np x F x K Float64, advanced in place), with the scratch in the library's fixed
+per-GPU array: no device memory is allocated (for code under graph capture or in serving
+loops).  Calls that overlap on one GPU race; sizes beyond the scratch are an error."""
+step_noalloc!(s::BBSolver, q::ROCArray{Float64}, dt::Real, nsteps::Integer; prec::Symbol = :mixed) =
+    check(ccall((:bb_solver_step_noalloc, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Cdouble, Cint), s.ptr,
+                prec_code(prec), reinterpret(Ptr{Cdouble}, pointer(q)), Float64(dt), Cint(nsteps)), s.ctx)
+
 # ------------------------------------------------------------------------- dense nodal comparator
 # extras/3d_nodal_wave.jl:57-73: dense products, flux directions and the combine
 

@@ -202,6 +202,21 @@ int bb_solver_set_tuning(bb_solver *s, int prec, int tiling);
  * in/out, neighbour traces, geometry, connectivity) and the DOF count K*Np*F */
 int bb_solver_stage_bytes(bb_solver *s, int prec, double *bytes, double *dofs);
 
+/* ---- stepping that allocates nothing -------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * fused path of bb_solver_step, with the state in the caller's device array q ([K][F][Np]
+ * FP64, advanced in place by nsteps steps of size dt) and the kernels' scratch -- the
+ * second state buffer of the stage ping-pong and the RK residual (FP64, or FP32 for
+ * BB_PREC_MIXED) -- in the library's fixed scratch array of BB_SCRATCH_BYTES per GPU.
+ * K*F*Np*8 bytes (rounded up to 256) plus K*F*Np*(8 or 4) bytes must fit in it, else
+ * BB_ERR_ARG.  No memory is allocated.  Every call on one GPU uses that GPU's single
+ * scratch array, which keeps the last call's contents: calls that overlap on one GPU (from
+ * other contexts or host threads) race.  Nothing in the scratch has to start at zero: the
+ * first stage of each step writes the residual before any stage reads it. */
+#define BB_SCRATCH_BYTES ((size_t)64 << 20)
+int bb_solver_step_noalloc(bb_solver *s, int prec, double *q, double dt, int nsteps);
+
 /* ---- dense nodal comparator (extras/3d_nodal_wave.jl:57-73) ------------------------
  * The reference's comparison RHS applies dense Np x Np derivative matrices and a dense
  * Np x 4Nfp lift to nodal values (mul!(dfxdr, Dr, fxu), ..., mul!(du, LIFT, flux, 1, -1)).
