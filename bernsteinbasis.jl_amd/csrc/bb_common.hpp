@@ -70,6 +70,7 @@ template <> struct DT<float> { static constexpr int code = BB_F32; };
 
 struct bb_ctx {
     int device = 0;
+    int cus = 0;  // compute units of the device (grid sizes of persistent kernels)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool capturing = false;

@@ -269,11 +269,13 @@ __global__ __launch_bounds__(kLiftW) void k_lift4w(const TI *__restrict__ x, TI 
 //                      in the slab (f1 slab j at (i,k), f2 slab l at (j,k), f3 slab i at
 //                      (j,k), f4 slab k at (i,j); src/BernsteinLift.jl:202-217)
 //   sl[n]            = the four slabs j | l << 4 | i << 8 | k << 12
+//   wt[4 n + f]      = lw_chain(N, slab of face f at n), faces f1 f2 f3 f4 (k_lift4x's gather)
 template <int N> struct LiftTab {
+    alignas(16) double wt[4 * np3(N)];
     uint16_t ab[np3(N)];
     uint16_t go[4 * np3(N)];
     uint16_t sl[np3(N)];
-    constexpr LiftTab() : ab{}, go{}, sl{} {
+    constexpr LiftTab() : wt{}, ab{}, go{}, sl{} {
         int r = 0;
         for (int d = 0; d <= N; ++d)
             for (int b = 0; b <= d; ++b)
@@ -288,6 +290,10 @@ template <int N> struct LiftTab {
                     go[4 * n + 2] = (uint16_t)(tet(N, i) + lin2(N - i, j, k));
                     go[4 * n + 3] = (uint16_t)(tet(N, k) + lin2(N - k, i, j));
                     sl[n] = (uint16_t)(j | (l << 4) | (i << 8) | (k << 12));
+                    wt[4 * n + 0] = lw_chain(N, j);
+                    wt[4 * n + 1] = lw_chain(N, l);
+                    wt[4 * n + 2] = lw_chain(N, i);
+                    wt[4 * n + 3] = lw_chain(N, k);
                 }
     }
 };
@@ -439,6 +445,233 @@ __global__ __launch_bounds__(kLiftR) void k_lift4r(const TI *__restrict__ x, TI 
 #pragma unroll
         for (int f = 0; f < F; ++f) oe[f * NP + n] = TI(acc[f]);
     }
+}
+
+// Compact field-major layouts, F as a template parameter, 5 <= N <= 9 (nfp(N) <= 64): face
+// data and every slab as LDS records and a final gather, as in k_lift4r, organised to issue
+// few instructions per byte, which is what bounds this lift (counters: the LDS costs about
+// the same time per instruction whatever its width, and once its reads are cut the VALU
+// issue binds).
+//  - Lane L keeps face node (a,b) = L of a triangle for the whole chain and its node's
+//    current value in registers: of the three points of a chain step only (a,b+1) is read
+//    from LDS, (a,b) is the lane's own value and (a+1,b) that of lane L+1, taken by a DPP
+//    wavefront shift.  L0 likewise takes (a-1,b) and (a+1,b) from lanes L-1 and L+1 and
+//    reads the other four stencil points.  A step is one record store and one record read
+//    (three reads and a store in k_lift4r).
+//  - The chains are packed as they shrink: one chain per wave while a step's output has
+//    more than 32 nodes, then two chains per wave, then all four in one wave (16-lane
+//    groups), each phase re-reading its first step's points from the slab records.
+//  - Two waves per element (faces w and w + 2 until the chains are packed), so that twice
+//    as many elements as with four waves are in flight where the LDS allows it.
+//  - The gather's slab weights come per node from the constant table.
+// Lanes outside a triangle, and lanes whose node has left the shrinking chain, compute
+// finite values that no lane in the chain reads with a non-zero weight and that are never
+// stored; their LDS reads go to a valid record.
+template <int N, int F, typename TI, typename TC> struct LiftXCfg {
+    static constexpr int NP = np3(N), NFP = nfp2(N);
+    static constexpr int W = LiftRCfg<N, F, TI, TC>::W;
+    static constexpr size_t lds = (size_t)4 * (NP + NFP) * F * sizeof(TC) + (N + 1) * 8;
+    static constexpr bool valid = N >= 5 && NFP <= 64 && lds <= 64 * 1024;
+};
+
+// chain steps of k_lift4x, packed as the chain shrinks: one chain per wave while a step's
+// output has more than 32 nodes, then two chains per wave (32-lane halves, waves 0 and 1),
+// then all four chains in wave 0 (16-lane groups) once it has at most 16
+template <int N> struct LiftXPhases {
+    static constexpr int last_step_over(int g) {
+        int s = 0;
+        while (s + 1 <= N && nfp2(N - s - 1) > g) ++s;
+        return s;
+    }
+    static constexpr int A = last_step_over(32);  // steps 1..A: one chain per wave
+    static constexpr int B = last_step_over(16);  // steps A+1..B: two chains per wave
+};
+
+// Steps s0..s1 of one chain in a group of lanes whose lane g holds node (a,b) = g of the
+// degree-(N - s0) triangle (the output of step s0): step s0 reads its three points from
+// the records of slab s0 - 1, later steps keep (a,b) in registers and take (a+1,b) from
+// lane g+1 (a DPP wavefront shift: an active lane's right neighbour is in its own group).
+template <int N, int F, int W, int S0, int S1, typename TC>
+__device__ __forceinline__ void lift_chain_group(TC *sw, int PSS, int g, const LiftTab<N> &tb) {
+    constexpr int d0 = N - S0;
+    const bool valid = g < nfp2(d0);
+    const int ab = valid ? tb.ab[np3(d0 - 1) + g] : 0;
+    const int a = ab & 255, b = ab >> 8;
+    const TC wa = TC(a + 1), wb = TC(b + 1);
+    TC wc = TC(d0 + 1 - a - b);
+    int idx = lin2(d0 + 1, a, b);  // (a,b) in the input triangle of the current step
+    TC xs[F];
+#pragma unroll
+    for (int s = S0; s <= S1; ++s) {
+        const int d = N - s;
+        const bool act = valid && a + b <= d;
+        const int o2 = act ? idx + d + 2 - b : 0;  // (a,b+1); inactive lanes read slab record 0
+        const TC *src = sw + tet(N, s - 1) * W;
+        TC x2[F];
+        if (s == S0) {
+            TC x0[F], x1[F];
+            plane_load<F, W>(src + __mul24(act ? idx : 0, W), PSS, x0);
+            plane_load<F, W>(src + __mul24(act ? idx + 1 : 0, W), PSS, x1);
+            plane_load<F, W>(src + __mul24(o2, W), PSS, x2);
+#pragma unroll
+            for (int f = 0; f < F; ++f) xs[f] = fma(wa, x1[f], fma(wb, x2[f], wc * x0[f]));
+        } else {
+            wave_lds_sync();
+            plane_load<F, W>(src + __mul24(o2, W), PSS, x2);
+#pragma unroll
+            for (int f = 0; f < F; ++f) xs[f] = fma(wa, xl_next(xs[f]), fma(wb, x2[f], wc * xs[f]));
+        }
+        if (act) plane_store<F, W>(sw + __mul24(tet(N, s) + idx - b, W), PSS, xs);
+        idx -= b;
+        wc -= TC(1);
+    }
+}
+
+template <int N, int F, typename TI, typename TC, typename TA, int NW>
+__global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI *__restrict__ out, int64_t K) {
+    constexpr int NP = np3(N), NFP = nfp2(N), W = LiftXCfg<N, F, TI, TC>::W;
+    constexpr int PSS = 4 * NP * W, PSX = 4 * NFP * W;  // plane strides
+    constexpr int PA = LiftXPhases<N>::A, PB = LiftXPhases<N>::B;
+    constexpr int FQ = 4 / NW;  // faces per wave until the chains are packed
+    static_assert(NFP <= 64 && (NW == 2 || NW == 4), "one lane per face node, 2 or 4 waves");
+    const LiftTab<N> &tb = kLiftTab<N>;
+    __shared__ __align__(16) TC S[4 * NP * F];   // [plane][face][slab node][W fields]
+    __shared__ __align__(16) TC X[4 * NFP * F];  // [plane][face][face node][W fields]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);  // faces w, w + NW, ...
+    const int64_t e = blockIdx.x;
+    const bool node = lane < NFP;
+    const int ab = node ? tb.ab[np3(N - 1) + lane] : 0;
+    const int a = ab & 255, b = ab >> 8;
+    // the wave's faces' data of every field: registers (one face node per lane) and LDS records
+    TC y[FQ][F];
+#pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+        const TI *xe = x + e * F * 4 * NFP + (w + q * NW) * NFP + (node ? lane : 0);
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            const TC v = TC(xe[f * 4 * NFP]);
+            y[q][f] = node ? v : TC(0);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < FQ; ++q)
+        if (node) plane_store<F, W>(X + __mul24((w + q * NW) * NFP + lane, W), PSX, y[q]);
+    wave_lds_sync();
+    // L0 (without its 1/2): the 7-point stencil of l0_prep_rec, same weights and order.
+    // (Values of lanes outside a triangle are never read with a non-zero weight: a lane in
+    // the chain reads only nodes that were in it one step earlier.  They stay finite -- a
+    // few steps of bounded weights -- and are never stored.)
+    const L0Rec<TC> pt = l0_prep_rec<N, W, TC>(a, b);
+#pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+        const int fc = w + q * NW;
+        const TC *xw = X + fc * NFP * W;
+        TC *sw = S + fc * NP * W;
+        TC xs[F];
+        {
+            TC y1[F], y3[F], y4[F], y6[F];
+            plane_load<F, W>(xw + pt.o1, PSX, y1);
+            plane_load<F, W>(xw + pt.o3, PSX, y3);
+            plane_load<F, W>(xw + pt.o4, PSX, y4);
+            plane_load<F, W>(xw + pt.o6, PSX, y6);
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                TC v = pt.w0 * y[q][f];
+                v = fma(pt.w1, y1[f], v);
+                v = fma(pt.w2, xl_next(y[q][f]), v);
+                v = fma(pt.w3, y3[f], v);
+                v = fma(pt.w4, y4[f], v);
+                v = fma(pt.w5, xl_prev(y[q][f]), v);
+                xs[f] = fma(pt.w6, y6[f], v);
+            }
+        }
+        if (node) plane_store<F, W>(sw + __mul24(lane, W), PSS, xs);
+        // the chain: (a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (d+1-a-b) x_{ab}
+        // (reduction_multiply! without /(d+1)).  Steps 1..PA: one chain per wave, lanes
+        // still on the degree-N layout.
+        if constexpr (PA >= 1) {
+            const int bF = __mul24(b, W);
+            int dstF = __mul24(lin2(N - 1, a, b), W);
+            int o2F = __mul24(lin2(N, a, b) + N + 1 - b, W);
+            TC wc = TC(N - a - b);
+            const TC wa = TC(a + 1), wb = TC(b + 1);
+#pragma unroll
+            for (int s = 1; s <= PA; ++s) {
+                const bool act = node && a + b <= N - s;
+                wave_lds_sync();
+                TC x2[F];
+                plane_load<F, W>(sw + tet(N, s - 1) * W + (act ? o2F : 0), PSS, x2);
+#pragma unroll
+                for (int f = 0; f < F; ++f) xs[f] = fma(wa, xl_next(xs[f]), fma(wb, x2[f], wc * xs[f]));
+                if (act) plane_store<F, W>(sw + tet(N, s) * W + dstF, PSS, xs);
+                dstF -= bF;
+                o2F -= bF + W;
+                wc -= TC(1);
+            }
+        }
+    }
+    // steps PA+1..PB: two chains per wave (waves 0, 1; chain 2 w + half)
+    if constexpr (PB > PA) {
+        __syncthreads();
+        if (w < 2)
+            lift_chain_group<N, F, W, PA + 1, PB, TC>(S + (2 * w + (lane >> 5)) * NP * W, PSS, lane & 31, tb);
+    }
+    // steps PB+1..N: the four chains in the 16-lane groups of wave 0
+    if constexpr (N > PB) {
+        __syncthreads();
+        if (w == 0) lift_chain_group<N, F, W, PB + 1, N, TC>(S + (lane >> 4) * NP * W, PSS, lane & 15, tb);
+    }
+    __syncthreads();
+    // gather: out[n] = G4 + G1 + G2 + G3 with the slab weights (a per-node table)
+    TI *oe = out + e * F * NP;
+    for (int n = tid; n < NP; n += 64 * NW) {
+        const uint2 gp = reinterpret_cast<const uint2 *>(tb.go)[n];
+        const int go[4] = {(int)(gp.x & 0xffff), (int)(gp.x >> 16), (int)(gp.y & 0xffff), (int)(gp.y >> 16)};
+        const double2 w01 = reinterpret_cast<const double2 *>(tb.wt)[2 * n];
+        const double2 w23 = reinterpret_cast<const double2 *>(tb.wt)[2 * n + 1];
+        const double wf[4] = {w01.x, w01.y, w23.x, w23.y};
+        TA acc[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) acc[f] = TA(0);
+        constexpr int order[4] = {3, 0, 1, 2};
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int ff = order[o];
+            const TA wt = TA(wf[ff]);
+            TC v[F];
+            plane_load<F, W>(S + ff * NP * W + __mul24(go[ff], W), PSS, v);
+#pragma unroll
+            for (int f = 0; f < F; ++f) acc[f] = fma(wt, TA(v[f]), acc[f]);
+        }
+#pragma unroll
+        for (int f = 0; f < F; ++f) oe[f * NP + n] = TI(acc[f]);
+    }
+}
+
+template <int N, typename TI, typename TC, typename TA>
+bool try_lift4x(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
+    constexpr int NP = np3(N), NFP = nfp2(N);
+    if (lx.elem != (int64_t)F * 4 * NFP || lx.node != 1 || (F > 1 && lx.field != 4 * NFP)) return false;
+    if (lo.elem != (int64_t)F * NP || lo.node != 1 || (F > 1 && lo.field != NP)) return false;
+    bool done = false;
+    auto one = [&](auto fc) {
+        constexpr int FF = decltype(fc)::value;
+        if constexpr (LiftXCfg<N, FF, TI, TC>::valid) {
+            if (F == FF) {
+                // two waves per element: twice the elements of four-wave blocks in flight where
+                // the LDS allows it (FP32 N=7 F=4: 518 -> 352 us), no slower elsewhere
+                k_lift4x<N, FF, TI, TC, TA, 2><<<(unsigned)K, 128, 0, ctx->stream>>>((const TI *)x, (TI *)out, K);
+                done = true;
+            }
+        }
+    };
+    one(std::integral_constant<int, 1>{});
+    one(std::integral_constant<int, 2>{});
+    one(std::integral_constant<int, 3>{});
+    one(std::integral_constant<int, 4>{});
+    one(std::integral_constant<int, 6>{});
+    return done;
 }
 
 template <int N, typename TI, typename TC, typename TA>
@@ -602,7 +835,8 @@ __global__ __launch_bounds__(kLiftThreads) void k_face_lift(const TI *__restrict
 
 template <int N, typename TI, typename TC, typename TA>
 int launch_lift4(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
-    if (K <= 0x7fffffff && (try_lift4r<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo) ||
+    if (K <= 0x7fffffff && (try_lift4x<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo) ||
+                            try_lift4r<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo) ||
                             try_lift4w<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo))) {
         BB_LAUNCH_CHECK(ctx);
         return BB_OK;

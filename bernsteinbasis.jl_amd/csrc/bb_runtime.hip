@@ -57,6 +57,7 @@ int bb_ctx_create(int device, bb_ctx **out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         scratch_address(device, &c->scratch) != hipSuccess) {
         if (c->ev1) hipEventDestroy(c->ev1);
         if (c->ev0) hipEventDestroy(c->ev0);

@@ -304,6 +304,35 @@ __host__ __device__ constexpr double lw_chain(int N, int s) {
     return lj(N, s) / d;
 }
 
+// Cross-lane moves within a wave (no LDS storage): the value of lane + 1 (DPP wavefront
+// shift left; 0 in lane 63), of lane - 1 (shift right; 0 in lane 0), and of the lane whose
+// index is addr / 4 (ds_bpermute; the lane index is taken modulo 64).  Every lane of the
+// wave must execute these with the same control flow (full exec), since a disabled source
+// lane does not supply its value.
+// (mov_dpp with bound_ctrl: a lane whose source is outside the wave gets 0, with no old value
+// to materialise)
+__device__ __forceinline__ int xl_next_i(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
+__device__ __forceinline__ int xl_prev_i(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int xl_from_i(int addr, int v) { return __builtin_amdgcn_ds_bpermute(addr, v); }
+__device__ __forceinline__ float xl_next(float v) { return __int_as_float(xl_next_i(__float_as_int(v))); }
+__device__ __forceinline__ float xl_prev(float v) { return __int_as_float(xl_prev_i(__float_as_int(v))); }
+__device__ __forceinline__ float xl_from(int addr, float v) { return __int_as_float(xl_from_i(addr, __float_as_int(v))); }
+__device__ __forceinline__ double xl_join(int lo, int hi) {
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double xl_next(double v) {
+    const long long u = __double_as_longlong(v);
+    return xl_join(xl_next_i((int)u), xl_next_i((int)(u >> 32)));
+}
+__device__ __forceinline__ double xl_prev(double v) {
+    const long long u = __double_as_longlong(v);
+    return xl_join(xl_prev_i((int)u), xl_prev_i((int)(u >> 32)));
+}
+__device__ __forceinline__ double xl_from(int addr, double v) {
+    const long long u = __double_as_longlong(v);
+    return xl_join(xl_from_i(addr, (int)u), xl_from_i(addr, (int)(u >> 32)));
+}
+
 // (E_{d}^{d+1})^T x: degree d+1 -> degree d at (a,b), reduction_multiply!
 // (src/BernsteinLift.jl:66-90): ((a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (d+1-a-b) x_{ab}) / (d+1)
 template <typename T> __device__ __forceinline__ T reduce_apply(const T *x, int d, int a, int b, T inv) {
