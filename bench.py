@@ -181,6 +181,17 @@ def operator_bench(ctx, reps=10):
         b = K * F * (4 * NFP + NP) * sz
         out[f"lift_{name}"] = {"us": ms * 1e3, "GBs": b / (ms * 1e-3) * 1e-9, "bytes": b,
                                "dof_per_s": K * F * NP / (ms * 1e-3)}
+        if dt == np.float32:
+            # the same lift with L0 on the matrix cores (FP32 MFMA), timed beside the VALU one
+            Lm = bbdg.BernsteinLift(N, dt, "acc64_mfma")
+            bbdg.mul_(lv, Lm, fv)
+            ctx.sync()
+            ctx.timer_start()
+            for _ in range(reps):
+                bbdg.mul_(lv, Lm, fv)
+            ms = ctx.timer_stop() / reps
+            out["lift_f32_mfma_l0"] = {"us": ms * 1e3, "GBs": b / (ms * 1e-3) * 1e-9, "bytes": b,
+                                       "dof_per_s": K * F * NP / (ms * 1e-3)}
         for a in (x, fd, lo):
             a.free()
     geo.free()

@@ -15,7 +15,7 @@ BB_OK, BB_ERR_ARG, BB_ERR_NULL, BB_ERR_HIP, BB_ERR_UNSUPPORTED, BB_ERR_STATE, BB
 BB_F64, BB_F32 = 0, 1
 BB_PDE_WAVE, BB_PDE_MAXWELL = 0, 1
 BB_FLUX_REFERENCE, BB_FLUX_UPWIND = 0, 1
-BB_LIFT_PLAIN, BB_LIFT_ACC64, BB_LIFT_TWOSUM = 0, 1, 2
+BB_LIFT_PLAIN, BB_LIFT_ACC64, BB_LIFT_TWOSUM, BB_LIFT_ACC64_MFMA = 0, 1, 2, 3
 BB_PREC_F64, BB_PREC_MIXED = 0, 1
 BB_PATH_PLAIN, BB_PATH_FUSED = 0, 1
 

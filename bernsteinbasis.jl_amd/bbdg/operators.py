@@ -150,7 +150,8 @@ class BernsteinDerivativeMatrix_3D_t(_Deriv):
 
 class BernsteinLift:
     """src/BernsteinLift.jl:12-27.  ``dtype`` float64 or float32; ``mode`` selects the FP32
-    arithmetic ('plain' or 'acc64': FP32 chain with FP64 weighted accumulation).  No
+    arithmetic ('plain'; 'acc64': FP32 chain with FP64 weighted accumulation; 'acc64_mfma': as
+    'acc64' with L0 on the matrix cores where that kernel exists).  No
     host scratch: the scratch vectors of the reference live in LDS."""
 
     def __init__(self, N, dtype=np.float64, mode="acc64"):
@@ -158,7 +159,8 @@ class BernsteinLift:
             raise ValueError("Supports 1 <= N <= 20")
         self.N = N
         self.dtype = np.dtype(dtype)
-        self.mode = {"plain": B.BB_LIFT_PLAIN, "acc64": B.BB_LIFT_ACC64, "twosum": B.BB_LIFT_TWOSUM}[mode]
+        self.mode = {"plain": B.BB_LIFT_PLAIN, "acc64": B.BB_LIFT_ACC64, "twosum": B.BB_LIFT_TWOSUM,
+                     "acc64_mfma": B.BB_LIFT_ACC64_MFMA}[mode]
 
     @property
     def size(self):

@@ -474,6 +474,39 @@ template <int N, int F, typename TI, typename TC> struct LiftXCfg {
     static constexpr bool valid = N >= 5 && NFP <= 64 && lds <= 64 * 1024;
 };
 
+// 2 L0 = (N+1)^2 (E_N^{N+1})^T E_N^{N+1}, the dense Nfp x Nfp matrix of the 7-point stencil
+// of l0_prep_rec (integer entries, exact in FP32), as A operands of v_mfma_f32_16x16x4f32:
+// row tile mt, lane L, k step kk holds L0[16 mt + L % 16][4 kk + L / 16] at a[(mt 64 + L) KT + kk]
+// (zero outside the matrix); klo/khi bound each row tile's k steps that hold non-zero entries.
+template <int N> struct L0MTab {
+    static constexpr int NFP = nfp2(N), MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
+    float a[MT * 64 * KT];
+    int klo[MT], khi[MT];
+    constexpr L0MTab() : a{}, klo{}, khi{} {
+        for (int mt = 0; mt < MT; ++mt) {
+            klo[mt] = KT;
+            khi[mt] = -1;
+        }
+        for (int b = 0; b <= N; ++b)
+            for (int aa = 0; aa <= N - b; ++aa) {
+                const int c = N - aa - b, r = lin2(N, aa, b);
+                const int pts[7][3] = {{aa, b, (aa + 1) * (aa + 1) + (b + 1) * (b + 1) + (c + 1) * (c + 1)},
+                                       {aa + 1, b - 1, (aa + 1) * b},     {aa + 1, b, (aa + 1) * c},
+                                       {aa - 1, b + 1, (b + 1) * aa},     {aa, b + 1, (b + 1) * c},
+                                       {aa - 1, b, (c + 1) * aa},         {aa, b - 1, (c + 1) * b}};
+                for (int q = 0; q < 7; ++q) {
+                    if (pts[q][2] == 0) continue;
+                    const int col = lin2(N, pts[q][0], pts[q][1]);
+                    const int mt = r / 16, kk = col / 4, L = (r % 16) + 16 * (col % 4);
+                    a[(mt * 64 + L) * KT + kk] += (float)pts[q][2];
+                    klo[mt] = kk < klo[mt] ? kk : klo[mt];
+                    khi[mt] = kk > khi[mt] ? kk : khi[mt];
+                }
+            }
+    }
+};
+template <int N> __device__ constexpr L0MTab<N> kL0MTab = L0MTab<N>();
+
 // chain steps of k_lift4x, packed as the chain shrinks: one chain per wave while a step's
 // output has more than 32 nodes, then two chains per wave (32-lane halves, waves 0 and 1),
 // then all four chains in wave 0 (16-lane groups) once it has at most 16
@@ -527,13 +560,14 @@ __device__ __forceinline__ void lift_chain_group(TC *sw, int PSS, int g, const L
     }
 }
 
-template <int N, int F, typename TI, typename TC, typename TA, int NW>
+template <int N, int F, typename TI, typename TC, typename TA, int NW, bool L0M = false>
 __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI *__restrict__ out, int64_t K) {
     constexpr int NP = np3(N), NFP = nfp2(N), W = LiftXCfg<N, F, TI, TC>::W;
     constexpr int PSS = 4 * NP * W, PSX = 4 * NFP * W;  // plane strides
     constexpr int PA = LiftXPhases<N>::A, PB = LiftXPhases<N>::B;
     constexpr int FQ = 4 / NW;  // faces per wave until the chains are packed
     static_assert(NFP <= 64 && (NW == 2 || NW == 4), "one lane per face node, 2 or 4 waves");
+    static_assert(!L0M || (std::is_same_v<TC, float> && F == 4), "matrix-core L0: FP32, 4 fields (16 columns)");
     const LiftTab<N> &tb = kLiftTab<N>;
     __shared__ __align__(16) TC S[4 * NP * F];   // [plane][face][slab node][W fields]
     __shared__ __align__(16) TC X[4 * NFP * F];  // [plane][face][face node][W fields]
@@ -562,6 +596,32 @@ __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI
     // (Values of lanes outside a triangle are never read with a non-zero weight: a lane in
     // the chain reads only nodes that were in it one step earlier.  They stay finite -- a
     // few steps of bounded weights -- and are never stored.)
+    if constexpr (L0M) {
+        // slab 0 of all four chains as one product 2 L0 X on the matrix cores: the element's
+        // 16 columns (face, field) of X are the B operand, read from the face records; wave
+        // w computes the row tiles mt = w, w + NW, .. over the k steps that hold non-zero
+        // entries of L0 (the stencil couples face nodes at most N+2 apart)
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        constexpr int MT = L0MTab<N>::MT, KT = L0MTab<N>::KT;
+        const L0MTab<N> &lt = kL0MTab<N>;
+        const int col = lane & 15, fc = col / F, fld = col % F;
+        __syncthreads();  // the face records of every wave
+        for (int mt = w; mt < MT; mt += NW) {
+            v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+            const float *ap = lt.a + (mt * 64 + lane) * KT;
+            for (int kk = lt.klo[mt]; kk <= lt.khi[mt]; ++kk) {
+                const int k = 4 * kk + (lane >> 4);
+                const float bv = k < NFP ? X[__mul24(fc * NFP + k, W) + fld] : 0.f;
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[kk], bv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * mt + 4 * (lane >> 4) + r;
+                if (row < NFP) S[__mul24(fc * NP + row, W) + fld] = acc[r];
+            }
+        }
+        __syncthreads();  // slab 0 of every chain
+    }
     const L0Rec<TC> pt = l0_prep_rec<N, W, TC>(a, b);
 #pragma unroll
     for (int q = 0; q < FQ; ++q) {
@@ -569,7 +629,11 @@ __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI
         const TC *xw = X + fc * NFP * W;
         TC *sw = S + fc * NP * W;
         TC xs[F];
-        {
+        if constexpr (L0M) {
+            // the chain continues from this lane's node of slab 0 (needed in registers only
+            // when the first steps run one chain per wave)
+            if constexpr (PA >= 1) plane_load<F, W>(sw + __mul24(lane, W), PSS, xs);
+        } else {
             TC y1[F], y3[F], y4[F], y6[F];
             plane_load<F, W>(xw + pt.o1, PSX, y1);
             plane_load<F, W>(xw + pt.o3, PSX, y3);
@@ -585,8 +649,8 @@ __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI
                 v = fma(pt.w5, xl_prev(y[q][f]), v);
                 xs[f] = fma(pt.w6, y6[f], v);
             }
+            if (node) plane_store<F, W>(sw + __mul24(lane, W), PSS, xs);
         }
-        if (node) plane_store<F, W>(sw + __mul24(lane, W), PSS, xs);
         // the chain: (a+1) x_{a+1,b} + (b+1) x_{a,b+1} + (d+1-a-b) x_{ab}
         // (reduction_multiply! without /(d+1)).  Steps 1..PA: one chain per wave, lanes
         // still on the degree-N layout.
@@ -649,7 +713,7 @@ __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI
     }
 }
 
-template <int N, typename TI, typename TC, typename TA>
+template <int N, typename TI, typename TC, typename TA, bool L0M = false>
 bool try_lift4x(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
     constexpr int NP = np3(N), NFP = nfp2(N);
     if (lx.elem != (int64_t)F * 4 * NFP || lx.node != 1 || (F > 1 && lx.field != 4 * NFP)) return false;
@@ -657,11 +721,11 @@ bool try_lift4x(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void
     bool done = false;
     auto one = [&](auto fc) {
         constexpr int FF = decltype(fc)::value;
-        if constexpr (LiftXCfg<N, FF, TI, TC>::valid) {
+        if constexpr (LiftXCfg<N, FF, TI, TC>::valid && (!L0M || FF == 4)) {
             if (F == FF) {
                 // two waves per element: twice the elements of four-wave blocks in flight where
                 // the LDS allows it (FP32 N=7 F=4: 518 -> 352 us), no slower elsewhere
-                k_lift4x<N, FF, TI, TC, TA, 2><<<(unsigned)K, 128, 0, ctx->stream>>>((const TI *)x, (TI *)out, K);
+                k_lift4x<N, FF, TI, TC, TA, 2, L0M><<<(unsigned)K, 128, 0, ctx->stream>>>((const TI *)x, (TI *)out, K);
                 done = true;
             }
         }
@@ -833,9 +897,9 @@ __global__ __launch_bounds__(kLiftThreads) void k_face_lift(const TI *__restrict
     }
 }
 
-template <int N, typename TI, typename TC, typename TA>
+template <int N, typename TI, typename TC, typename TA, bool L0M = false>
 int launch_lift4(bb_ctx *ctx, int F, int64_t K, const void *x, bb_layout lx, void *out, bb_layout lo) {
-    if (K <= 0x7fffffff && (try_lift4x<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo) ||
+    if (K <= 0x7fffffff && (try_lift4x<N, TI, TC, TA, L0M>(ctx, F, K, x, lx, out, lo) ||
                             try_lift4r<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo) ||
                             try_lift4w<N, TI, TC, TA>(ctx, F, K, x, lx, out, lo))) {
         BB_LAUNCH_CHECK(ctx);
@@ -881,6 +945,7 @@ int bb_lift_apply(bb_ctx *ctx, int N, int F, int64_t K, int dtype, int mode, con
         if (dtype != BB_F32) return bb_fail(ctx, BB_ERR_ARG, "dtype %d", dtype);
         if (mode == BB_LIFT_PLAIN) return launch_lift4<NN, float, float, float>(ctx, F, K, x, lx, out, lo);
         if (mode == BB_LIFT_ACC64) return launch_lift4<NN, float, float, double>(ctx, F, K, x, lx, out, lo);
+        if (mode == BB_LIFT_ACC64_MFMA) return launch_lift4<NN, float, float, double, true>(ctx, F, K, x, lx, out, lo);
         if (mode == BB_LIFT_TWOSUM)
             return bb_fail(ctx, BB_ERR_UNSUPPORTED, "two-sum mode is implemented for the single-face lift only");
         return bb_fail(ctx, BB_ERR_ARG, "lift mode %d", mode);
@@ -896,7 +961,7 @@ int bb_face_lift_apply(bb_ctx *ctx, int N, int F, int64_t K, int x_dtype, int ou
         return bb_fail(ctx, BB_ERR_ARG, "negative stride");
     if ((x_dtype != BB_F64 && x_dtype != BB_F32) || (out_dtype != BB_F64 && out_dtype != BB_F32))
         return bb_fail(ctx, BB_ERR_ARG, "dtype %d/%d", x_dtype, out_dtype);
-    if (mode < BB_LIFT_PLAIN || mode > BB_LIFT_TWOSUM) return bb_fail(ctx, BB_ERR_ARG, "lift mode %d", mode);
+    if (mode < BB_LIFT_PLAIN || mode > BB_LIFT_ACC64_MFMA) return bb_fail(ctx, BB_ERR_ARG, "lift mode %d", mode);
     if (K == 0) return BB_OK;
     return dispatch_n<1, 20>(N, [&](auto nc) -> int {
         constexpr int NN = decltype(nc)::value;
@@ -904,7 +969,8 @@ int bb_face_lift_apply(bb_ctx *ctx, int N, int F, int64_t K, int x_dtype, int ou
             using TO = decltype(to);
             if (x_dtype == BB_F64) return launch_face_lift<NN, double, double, double, TO>(ctx, F, K, x, lx, out, lo);
             if (mode == BB_LIFT_PLAIN) return launch_face_lift<NN, float, float, float, TO>(ctx, F, K, x, lx, out, lo);
-            if (mode == BB_LIFT_ACC64) return launch_face_lift<NN, float, float, double, TO>(ctx, F, K, x, lx, out, lo);
+            if (mode == BB_LIFT_ACC64 || mode == BB_LIFT_ACC64_MFMA)
+                return launch_face_lift<NN, float, float, double, TO>(ctx, F, K, x, lx, out, lo);
             return launch_face_lift<NN, float, float, FF, TO>(ctx, F, K, x, lx, out, lo);
         };
         return out_dtype == BB_F64 ? go(double(0)) : go(float(0));

@@ -53,6 +53,7 @@ const BB_AXIS_R = Cint(0); const BB_AXIS_S = Cint(1); const BB_AXIS_T = Cint(2)
 const BB_PDE_WAVE = Cint(0); const BB_PDE_MAXWELL = Cint(1)
 const BB_FLUX_REFERENCE = Cint(0); const BB_FLUX_UPWIND = Cint(1)
 const BB_LIFT_PLAIN = Cint(0); const BB_LIFT_ACC64 = Cint(1); const BB_LIFT_TWOSUM = Cint(2)
+const BB_LIFT_ACC64_MFMA = Cint(3)
 const BB_PREC_F64 = Cint(0); const BB_PREC_MIXED = Cint(1)
 const BB_PATH_PLAIN = Cint(0); const BB_PATH_FUSED = Cint(1)
 

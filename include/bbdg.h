@@ -47,8 +47,10 @@ enum bb_pde { BB_PDE_WAVE = 0, BB_PDE_MAXWELL = 1 };
  * terms scaled by the face Jacobian Jf = |nJ| (stable under explicit RK on fine meshes) */
 enum bb_flux { BB_FLUX_REFERENCE = 0, BB_FLUX_UPWIND = 1 };
 /* lift arithmetic for FP32 data: everything FP32; FP32 chain + FP64 weighted
- * accumulation; FP32 chain carried as float-float (compensated, two-sum) */
-enum bb_lift_mode { BB_LIFT_PLAIN = 0, BB_LIFT_ACC64 = 1, BB_LIFT_TWOSUM = 2 };
+ * accumulation; FP32 chain carried as float-float (compensated, two-sum); as ACC64 with
+ * L0 on the matrix cores (FP32 MFMA, FP32 accumulation) where a kernel for it is built
+ * (4-face lift, compact field-major layout, F = 4, 5 <= N <= 9), else as ACC64 */
+enum bb_lift_mode { BB_LIFT_PLAIN = 0, BB_LIFT_ACC64 = 1, BB_LIFT_TWOSUM = 2, BB_LIFT_ACC64_MFMA = 3 };
 /* solver arithmetic: FP64 everywhere; mixed = FP32 derivative/flux/L0/chain, FP64 lift
  * accumulation and FP64 solution accumulator (FP32 RK residual register) */
 enum bb_prec { BB_PREC_F64 = 0, BB_PREC_MIXED = 1 };

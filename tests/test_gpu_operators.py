@@ -176,7 +176,7 @@ def test_lift_field_counts(ctx, orc, N, F):
         _, ov = empty_cols(np3(N), K, F, layout, np.float64)
         bbdg.mul_(ov, bbdg.BernsteinLift(N), xv)
         assert rel(from_dev_cols(ov), ref) <= TOL64, (layout, rel(from_dev_cols(ov), ref))
-        for mode in ("plain", "acc64"):
+        for mode in ("plain", "acc64", "acc64_mfma"):
             _, xv = to_dev_cols(x, layout, np.float32)
             _, ov = empty_cols(np3(N), K, F, layout, np.float32)
             bbdg.mul_(ov, bbdg.BernsteinLift(N, np.float32, mode), xv)
@@ -184,7 +184,7 @@ def test_lift_field_counts(ctx, orc, N, F):
 
 
 @pytest.mark.parametrize("N", [2, 5, 7, 9])
-@pytest.mark.parametrize("mode", ["plain", "acc64"])
+@pytest.mark.parametrize("mode", ["plain", "acc64", "acc64_mfma"])
 def test_lift_fp32(ctx, orc, N, mode):
     import bbdg
 

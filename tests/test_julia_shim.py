@@ -170,7 +170,7 @@ def test_reference_methods_have_device_methods():
 
 @pytest.mark.parametrize("const", ["BB_F64", "BB_F32", "BB_AXIS_R", "BB_AXIS_S", "BB_AXIS_T", "BB_PDE_WAVE",
                                    "BB_PDE_MAXWELL", "BB_FLUX_REFERENCE", "BB_FLUX_UPWIND", "BB_LIFT_PLAIN",
-                                   "BB_LIFT_ACC64", "BB_LIFT_TWOSUM", "BB_PREC_F64", "BB_PREC_MIXED",
+                                   "BB_LIFT_ACC64", "BB_LIFT_TWOSUM", "BB_LIFT_ACC64_MFMA", "BB_PREC_F64", "BB_PREC_MIXED",
                                    "BB_PATH_PLAIN", "BB_PATH_FUSED"])
 def test_enum_values_match(const):
     text = _strip_c_comments(open(HEADER).read())
