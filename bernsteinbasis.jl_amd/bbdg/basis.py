@@ -52,16 +52,71 @@ def bernstein_basis_tet(N, r, s, t):
     return bernstein_basis_from_barycentric_tet(N, *cartesian_to_barycentric_tet(r, s, t))
 
 
+def evaluate_bernstein_derivatives(N, lam, a):
+    """src/utility.jl:109-138 (Tet) and :140-147 (Tri, the Tet formula with the fourth
+    coordinate 0 and exponent 0): the partial derivatives of the Bernstein polynomial of
+    multi-index a (3 or 4 exponents summing to N) with respect to each barycentric
+    coordinate, d/dlam_m = N!/(a - e_m)! prod lam^(a - e_m) (zero where a_m = 0), as a list
+    of arrays over the points.  A multi-index whose exponents do not sum to N raises
+    ValueError (the reference's DomainError)."""
+    a = tuple(int(v) for v in a)
+    if sum(a) != N or len(a) != len(lam) or min(a) < 0:
+        raise ValueError(f"barycentric exponents {a} do not sum to the degree {N}")
+    fN = factorial(N)
+    out = []
+    for m in range(len(a)):
+        if a[m] == 0:
+            out.append(np.zeros_like(lam[0]))
+            continue
+        b = list(a)
+        b[m] -= 1
+        coef = fN / float(np.prod([factorial(v) for v in b]))
+        val = np.full_like(lam[0], coef)
+        for c, e in zip(lam, b):
+            val = val * c ** e
+        out.append(val)
+    return out
+
+
+def cartesian_to_barycentric_line(r):
+    """src/utility.jl:8-13: (r + 1)/2, (1 - r)/2"""
+    r = np.asarray(r, dtype=np.float64)
+    return (r + 1) / 2, (1 - r) / 2
+
+
+def bernstein_basis_from_barycentric_line(N, l0, l1):
+    """src/utility.jl:83-86: V[m, i] = C(N, i) l0^i l1^(N - i), i = 0..N; no derivative
+    columns (the reference returns nothing for them)"""
+    l0, l1 = (np.asarray(v, dtype=np.float64) for v in (l0, l1))
+    fN = factorial(N)
+    V = np.stack([fN / (factorial(i) * factorial(N - i)) * l0 ** i * l1 ** (N - i) for i in range(N + 1)], axis=1)
+    return V, None
+
+
+def bernstein_basis_line(N, r):
+    """src/utility.jl:56-58"""
+    return bernstein_basis_from_barycentric_line(N, *cartesian_to_barycentric_line(r))
+
+
 def bernstein_basis_from_barycentric_tri(N, l0, l1, l2):
-    """src/utility.jl:88-95 (V only): columns (j outer, i inner)"""
+    """src/utility.jl:88-95: V[m, n] = psi_n(point m), columns (j outer, i inner), and the
+    barycentric derivative columns Vi, Vj, Vk (src/utility.jl:140-147)."""
     l0, l1, l2 = (np.asarray(v, dtype=np.float64) for v in (l0, l1, l2))
     fN = factorial(N)
     cols = []
+    D = [[], [], []]
     for j in range(N + 1):
         for i in range(N + 1 - j):
             k = N - i - j
             cols.append(fN / (factorial(i) * factorial(j) * factorial(k)) * l0 ** i * l1 ** j * l2 ** k)
-    return np.stack(cols, axis=1)
+            for m, d in enumerate(evaluate_bernstein_derivatives(N, (l0, l1, l2), (i, j, k))):
+                D[m].append(d)
+    return (np.stack(cols, axis=1), *(np.stack(d, axis=1) for d in D))
+
+
+def bernstein_basis_tri(N, r, s):
+    """src/utility.jl:60-61"""
+    return bernstein_basis_from_barycentric_tri(N, *cartesian_to_barycentric_tri(r, s))
 
 
 def equispaced_ref_points(N):

@@ -57,7 +57,7 @@ class NodalOperators:
         x.free()
         out.free()
         abc = face_node_abc(N).astype(np.float64) / N
-        Vf = bernstein_basis_from_barycentric_tri(N, abc[:, 0], abc[:, 1], abc[:, 2])
+        Vf = bernstein_basis_from_barycentric_tri(N, abc[:, 0], abc[:, 1], abc[:, 2])[0]
         Vf_inv = np.linalg.inv(Vf)
         self.LIFT = V @ LB @ np.kron(np.eye(4), Vf_inv)
 
