@@ -5,13 +5,15 @@ fallback: if it is missing, every compute entry point raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SO = os.path.join(os.path.dirname(HERE), "libbbdg.so")
 
-BB_OK, BB_ERR_ARG, BB_ERR_NULL, BB_ERR_HIP, BB_ERR_UNSUPPORTED, BB_ERR_STATE, BB_ERR_NOMEM = range(7)
+BB_OK, BB_ERR_ARG, BB_ERR_NULL, BB_ERR_HIP, BB_ERR_UNSUPPORTED, BB_ERR_STATE, BB_ERR_NOMEM, BB_ERR_NONFINITE = range(8)
 BB_F64, BB_F32 = 0, 1
 BB_PDE_WAVE, BB_PDE_MAXWELL = 0, 1
 BB_FLUX_REFERENCE, BB_FLUX_UPWIND = 0, 1
@@ -20,7 +22,7 @@ BB_PREC_F64, BB_PREC_MIXED = 0, 1
 BB_PATH_PLAIN, BB_PATH_FUSED = 0, 1
 
 ERR_NAMES = {1: "BB_ERR_ARG", 2: "BB_ERR_NULL", 3: "BB_ERR_HIP", 4: "BB_ERR_UNSUPPORTED", 5: "BB_ERR_STATE",
-             6: "BB_ERR_NOMEM"}
+             6: "BB_ERR_NOMEM", 7: "BB_ERR_NONFINITE"}
 
 
 class bb_layout(C.Structure):
@@ -52,6 +54,7 @@ SIGS = {
     "bb_fill_zero": (i32, [vp, vp, C.c_size_t]),
     "bb_stream_copy": (i32, [vp, vp, vp, C.c_size_t]),
     "bb_synchronize": (i32, [vp]),
+    "bb_count_nonfinite": (i32, [vp, i32, vp, i64, C.POINTER(C.c_int64)]),
     "bb_timer_start": (i32, [vp]),
     "bb_timer_stop": (i32, [vp, C.POINTER(C.c_float)]),
     "bb_tables": (i32, [i32, vp, vp, vp]),
@@ -72,6 +75,7 @@ SIGS = {
     "bb_solver_set_state": (i32, [vp, vp]),
     "bb_solver_get_state": (i32, [vp, vp]),
     "bb_solver_state_ptr": (i32, [vp, C.POINTER(vp)]),
+    "bb_solver_check_state": (i32, [vp, C.POINTER(C.c_int64)]),
     "bb_solver_rhs": (i32, [vp, i32, i32, vp]),
     "bb_solver_step": (i32, [vp, i32, i32, dbl, i32]),
     "bb_solver_graph_capture": (i32, [vp, i32, i32, dbl, i32]),
@@ -88,6 +92,7 @@ SIGS = {
     "bb_batch_state_words": (i32, [vp, C.POINTER(C.c_int64)]),
     "bb_batch_set_state": (i32, [vp, vp]),
     "bb_batch_get_state": (i32, [vp, vp]),
+    "bb_batch_check_state": (i32, [vp, C.POINTER(C.c_int64)]),
     "bb_batch_rhs": (i32, [vp, i32, vp]),
     "bb_batch_step": (i32, [vp, i32, dbl, i32]),
     "bb_batch_graph_capture": (i32, [vp, i32, dbl, i32]),
@@ -97,6 +102,28 @@ SIGS = {
 }
 
 _lib = None
+
+# Library objects still open when the interpreter exits are released by an exit hook, while
+# the HIP runtime is still up: arrays, solvers and batches first, then their contexts.
+# (Left to garbage collection at the very end of the process, a release could run after
+# the runtime's own teardown.)
+_open_objs = weakref.WeakSet()
+_open_ctxs = weakref.WeakSet()
+
+
+def track(obj, context=False):
+    (_open_ctxs if context else _open_objs).add(obj)
+    return obj
+
+
+@atexit.register
+def _release_all():
+    for group, meth in ((_open_objs, None), (_open_ctxs, "close")):
+        for o in list(group):
+            try:
+                getattr(o, meth or ("free" if hasattr(o, "free") else "close"))()
+            except Exception:
+                pass
 
 
 def lib():

@@ -136,6 +136,7 @@ class BatchSolver:
             p(batch.perm, C.c_int32), p(batch.bc, C.c_int32), p(np.ascontiguousarray(sign), C.c_double),
             p(batch.dte, C.c_double), p(self._T, C.c_double), p(self._off, C.c_int64), self._T.size, C.byref(h)))
         self.h = h
+        B.track(self)
 
     def close(self):
         if self.h:
@@ -158,6 +159,14 @@ class BatchSolver:
         q = np.empty(self.batch.words)
         self.ctx.check(B.lib().bb_batch_get_state(self.h, q.ctypes.data_as(C.POINTER(C.c_double))))
         return q
+
+    def check_state(self):
+        """number of NaN / Inf values in the device state of every degree group"""
+        n = C.c_int64()
+        rc = B.lib().bb_batch_check_state(self.h, C.byref(n))
+        if rc not in (B.BB_OK, B.BB_ERR_NONFINITE):
+            self.ctx.check(rc)
+        return n.value
 
     def rhs(self, prec="f64"):
         du = np.empty(self.batch.words)

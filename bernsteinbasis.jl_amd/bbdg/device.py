@@ -22,6 +22,7 @@ class Context:
             raise B.BBError(rc, f"cannot create a context on device {device} (is a GPU visible?)")
         self.h = h
         self.device = device
+        B.track(self, context=True)
 
     def close(self):
         if self.h:
@@ -101,6 +102,7 @@ class DeviceArray:
             self.ptr = p.value
             self._owner = None
             self._owns = True
+            B.track(self)
         else:
             self.ptr = _ptr
             self._owner = _owner
@@ -162,6 +164,14 @@ class DeviceArray:
         self.ctx.check(B.lib().bb_memcpy_d2h(self.ctx.h, buf.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
                                              buf.nbytes))
         return np.lib.stride_tricks.as_strided(buf, self.shape, [s * self.dtype.itemsize for s in self.strides]).copy()
+
+    def count_nonfinite(self) -> int:
+        """number of NaN / Inf values in the array's memory span (a device scan)"""
+        n = C.c_int64()
+        rc = B.lib().bb_count_nonfinite(self.ctx.h, self.code, C.c_void_p(self.ptr), self._span(), C.byref(n))
+        if rc not in (B.BB_OK, B.BB_ERR_NONFINITE):
+            self.ctx.check(rc)
+        return n.value
 
     def view(self, shape, strides) -> "DeviceArray":
         """A re-strided view of the same memory (no copy, no bounds growth)."""

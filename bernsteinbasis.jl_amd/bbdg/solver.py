@@ -49,6 +49,7 @@ class Solver:
         self.ctx.check(B.lib().bb_solver_create(self.ctx.h, PDES[pde], FLUX[flux], N, self.K,
                                                 *[a.ctypes.data_as(C.c_void_p) for a in self._keep], C.byref(h)))
         self.h = h
+        B.track(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -73,6 +74,14 @@ class Solver:
         soa = np.empty((self.K, self.F, self.Np))
         self.ctx.check(B.lib().bb_solver_get_state(self.h, soa.ctypes.data_as(C.c_void_p)))
         return np.ascontiguousarray(soa.transpose(0, 2, 1))
+
+    def check_state(self):
+        """number of NaN / Inf values in the device state (0 while the run is healthy)"""
+        n = C.c_int64()
+        rc = B.lib().bb_solver_check_state(self.h, C.byref(n))
+        if rc not in (B.BB_OK, B.BB_ERR_NONFINITE):
+            self.ctx.check(rc)
+        return n.value
 
     def rhs(self, path="fused", prec="f64"):
         soa = np.empty((self.K, self.F, self.Np))

@@ -38,7 +38,7 @@ struct bb_batch {
     int32_t *erow = nullptr;
     uint8_t *ecol = nullptr;
     int2 *conn = nullptr;
-    int64_t *nboff = nullptr, *troff = nullptr, *eoff = nullptr;
+    int64_t *nboff = nullptr;
     double *q[2] = {nullptr, nullptr};
     int cur = 0;
     double *res64 = nullptr;
@@ -75,11 +75,9 @@ static StageArgs batch_args(bb_batch *b, const BatchGroup &g, int prec, int cur)
     a.q_nb = b->q[cur];
     a.nboff = b->nboff + g.start * 4;
     a.tr = b->tr;
-    a.troff = b->troff;
     a.dte = b->dte + g.start;
     a.trf = b->trf;
     a.erow = b->erow;
-    a.eoff = b->eoff;
     a.ecol = b->ecol;
     a.evd = b->evd;
     a.evf = b->evf;
@@ -199,16 +197,15 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
     };
     hipError_t err;
     const int64_t words = h.words;
-    const int64_t tl = transfer_len > 0 ? transfer_len : 1;
+    const int64_t tl = transfer_len + kTransferPad;
     if ((err = hipMalloc(&b->geo, K * 9 * sizeof(double))) || (err = hipMalloc(&b->invJ, K * sizeof(double))) ||
         (err = hipMalloc(&b->geo32, K * 9 * sizeof(float))) ||
         (err = hipMalloc(&b->nJ, K * 20 * sizeof(double))) || (err = hipMalloc(&b->dte, K * sizeof(double))) ||
         (err = hipMalloc(&b->conn, K * 4 * sizeof(int2))) || (err = hipMalloc(&b->nboff, K * 4 * sizeof(int64_t))) ||
         (err = hipMalloc(&b->tr, tl * sizeof(double))) ||
-        (err = hipMalloc(&b->troff, 121 * sizeof(int64_t))) || (err = hipMalloc(&b->q[0], words * sizeof(double))) ||
+        (err = hipMalloc(&b->q[0], words * sizeof(double))) ||
         (err = hipMalloc(&b->trf, tl * sizeof(float))) ||
         (err = hipMalloc(&b->erow, (h.erow.size() + 1) * sizeof(int32_t))) ||
-        (err = hipMalloc(&b->eoff, 121 * sizeof(int64_t))) ||
         (err = hipMalloc(&b->ecol, h.ecol.size() + 1)) ||
         (err = hipMalloc(&b->evd, (h.evd.size() + 1) * sizeof(double))) ||
         (err = hipMalloc(&b->evf, (h.evf.size() + 1) * sizeof(float))) ||
@@ -222,14 +219,19 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
         (err = hipMemcpy(b->conn, h.conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->nboff, h.nboff.data(), K * 4 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->tr, transfer_host, transfer_len * sizeof(double), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->troff, transfer_off_host, 121 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->trf, h.trf.data(), h.trf.size() * sizeof(float), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->erow, h.erow.data(), h.erow.size() * sizeof(int32_t), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->eoff, h.eoff.data(), 121 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->ecol, h.ecol.data(), h.ecol.size(), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->evd, h.evd.data(), h.evd.size() * sizeof(double), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->evf, h.evf.data(), h.evf.size() * sizeof(float), hipMemcpyHostToDevice)))
         return fail(err);
+    {
+        const int rc = bb_fill_zero(ctx, b->tr + transfer_len, kTransferPad * sizeof(double));
+        if (rc) {
+            bb_batch_destroy(b);
+            return rc;
+        }
+    }
     for (size_t i = 0; i < b->groups.size(); ++i) {
         BatchGroup &g = b->groups[i];
         const std::vector<uint16_t> &tab = tabs[i];
@@ -253,8 +255,8 @@ int bb_batch_destroy(bb_batch *b) {
     if (!b) return BB_ERR_NULL;
     hipStreamSynchronize(b->ctx->stream);
     batch_drop_graphs(b);
-    void *ptrs[] = {b->geo, b->geo32, b->invJ, b->nJ, b->dte, b->tr, b->conn, b->nboff, b->troff, b->q[0], b->q[1], b->res64, b->res32,
-                    b->trf, b->erow, b->eoff, b->ecol, b->evd, b->evf};
+    void *ptrs[] = {b->geo, b->geo32, b->invJ, b->nJ, b->dte, b->tr, b->conn, b->nboff, b->q[0], b->q[1], b->res64, b->res32,
+                    b->trf, b->erow, b->ecol, b->evd, b->evf};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &g : b->groups) {
@@ -273,6 +275,8 @@ int bb_batch_state_words(bb_batch *b, int64_t *words) {
 
 int bb_batch_set_state(bb_batch *b, const double *q_host) {
     if (!b || !q_host) return BB_ERR_NULL;
+    int rc = check_finite_host(b->ctx, "state", q_host, b->words);
+    if (rc) return rc;
     BB_HIP(b->ctx, hipMemcpyAsync(b->q[b->cur], q_host, b->words * sizeof(double), hipMemcpyHostToDevice,
                                   b->ctx->stream));
     BB_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
@@ -284,6 +288,14 @@ int bb_batch_get_state(bb_batch *b, double *q_host) {
     BB_HIP(b->ctx, hipMemcpyAsync(q_host, b->q[b->cur], b->words * sizeof(double), hipMemcpyDeviceToHost,
                                   b->ctx->stream));
     BB_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+    return BB_OK;
+}
+
+int bb_batch_check_state(bb_batch *b, int64_t *count) {
+    if (!b || !count) return BB_ERR_NULL;
+    const int rc = bb_nonfinite_device(b->ctx, BB_F64, b->q[b->cur], b->words, count);
+    if (rc) return rc;
+    if (*count) return bb_fail(b->ctx, BB_ERR_NONFINITE, "the state holds %lld NaN or Inf values", (long long)*count);
     return BB_OK;
 }
 

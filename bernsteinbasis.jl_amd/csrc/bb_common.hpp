@@ -79,6 +79,7 @@ struct bb_ctx {
     // address, taken once per device at context creation, and its size in bytes
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    unsigned long long *counter = nullptr;  // device counter of bb_count_nonfinite, allocated on first use
     char err[1024] = {0};  // message of the last failure (fixed size: recording it never allocates)
 };
 
@@ -118,6 +119,10 @@ inline int launch_status() {
     return t_launch_err == hipSuccess ? BB_OK : BB_ERR_HIP;
 }
 }  // namespace bb
+
+// count of NaN / Inf values among n values of dtype at device pointer x, into *count
+// (bb_runtime.hip; waits for the context's stream)
+int bb_nonfinite_device(bb_ctx *ctx, int dtype, const void *x, int64_t n, int64_t *count);
 
 // releases the context's library GEMM handle (bb_dense.hip)
 void bb_blas_release(bb_ctx *ctx);

@@ -29,6 +29,20 @@ inline bool face_scalings(const double *n, double *o) {
     return true;
 }
 
+// index of the first NaN or Inf among x[0 .. n), or -1
+inline int64_t first_nonfinite(const double *x, int64_t n) {
+    for (int64_t t = 0; t < n; ++t)
+        if (!isfinite(x[t])) return t;
+    return -1;
+}
+
+// refuses a host array holding a NaN or Inf (what = its name in the message)
+inline int check_finite_host(bb_ctx *ctx, const char *what, const double *x, int64_t n) {
+    const int64_t t = first_nonfinite(x, n);
+    if (t >= 0) return bb_fail(ctx, BB_ERR_NONFINITE, "%s[%lld] is not finite (%g)", what, (long long)t, x[t]);
+    return BB_OK;
+}
+
 struct SolverHost {
     int F = 0, NP = 0, NFP = 0;
     std::vector<int2> conn;                // [K][4]
@@ -53,6 +67,9 @@ inline int solver_host_setup(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t
     if (rc) return rc;
     if (K < 1) return bb_fail(ctx, BB_ERR_ARG, "K must be >= 1");
     if (!geo || !J || !nJ || !nbr || !nbr_face || !perm || !bc) return bb_fail(ctx, BB_ERR_NULL, "null mesh array");
+    if ((rc = check_finite_host(ctx, "geo", geo, K * 9)) || (rc = check_finite_host(ctx, "J", J, K)) ||
+        (rc = check_finite_host(ctx, "nJ", nJ, K * 12)))
+        return rc;
     h.NP = np3(N);
     h.NFP = nfp2(N);
     h.conn.resize(K * 4);
@@ -107,6 +124,9 @@ struct BatchHost {
     std::vector<float> evf;
 };
 
+// zero entries past the end of the device transfer tables (projection rows are read ahead)
+constexpr int kTransferPad = 8;
+
 // is the transfer matrix from degree Nn to N present in a table of transfer_len doubles?
 inline bool transfer_present(const int64_t *toff, int64_t transfer_len, int N, int Nn) {
     const int64_t o = toff[11 * N + Nn];
@@ -127,6 +147,11 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
     if (transfer_len < 0) return bb_fail(ctx, BB_ERR_ARG, "transfer_len %lld < 0", (long long)transfer_len);
     if (!deg || !geo || !J || !nJ || !nbr || !nbr_face || !perm || !bc || !dt || !transfer || !toff)
         return bb_fail(ctx, BB_ERR_NULL, "null batch array");
+    int rc;
+    if ((rc = check_finite_host(ctx, "geo", geo, K * 9)) || (rc = check_finite_host(ctx, "J", J, K)) ||
+        (rc = check_finite_host(ctx, "nJ", nJ, K * 12)) || (rc = check_finite_host(ctx, "dt", dt, K)) ||
+        (rc = check_finite_host(ctx, "transfer", transfer, transfer_len)))
+        return rc;
     const int F = h.F;
     std::vector<int64_t> qoff_e(K);
     int64_t words = 0;
@@ -183,7 +208,23 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
             h.erow.push_back((int32_t)h.evd.size());
         }
     h.evf.assign(h.evd.begin(), h.evd.end());
+    // conn.x of an interior face whose neighbour has another degree holds where that face's
+    // transfer rows start (the kernels reach each element's neighbour through nboff): the
+    // first entry of the projection matrix in the transfer table for a higher neighbour
+    // degree, the first row of the sparse elevation rows for a lower one; 0 otherwise
+    for (int64_t t = 0; t < K * 4; ++t) {
+        const int N = deg[t / 4], Nn = deg[nbr[t]];
+        int64_t base = 0;
+        if (!bc[t] && Nn > N) base = toff[11 * N + Nn];
+        if (!bc[t] && Nn < N) base = h.eoff[11 * N + Nn];
+        if (base < 0 || base > INT32_MAX)
+            return bb_fail(ctx, BB_ERR_ARG, "transfer table offset %lld out of range", (long long)base);
+        h.conn[t].x = (int)base;
+    }
+    // the table in FP32, with kTransferPad zeros past its end (the kernels' projection rows
+    // are read four entries ahead)
     h.trf.assign(transfer, transfer + transfer_len);
+    h.trf.resize(transfer_len + kTransferPad, 0.f);
     h.invJ.resize(K);
     h.geoh.resize(K * 9);
     h.geoh32.resize(K * 9);

@@ -75,6 +75,7 @@ int bb_ctx_destroy(bb_ctx *ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     bb_blas_release(ctx);
+    if (ctx->counter) hipFree(ctx->counter);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
     hipStreamDestroy(ctx->stream);
@@ -153,7 +154,50 @@ __global__ void bb_stream_copy_kernel(const uint4 *__restrict__ src, uint4 *__re
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// *count += number of NaN / Inf among x[0 .. n): a grid-stride scan, each wave's count summed
+// by ballot and one atomic per wave
+template <typename T> __global__ void k_count_nonfinite(const T *__restrict__ x, int64_t n, unsigned long long *count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned c = 0;
+    for (; i < n; i += stride) c += !isfinite(x[i]);
+    // wave sum by a butterfly over the 64 lanes
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
+int bb_nonfinite_device(bb_ctx *ctx, int dtype, const void *x, int64_t n, int64_t *count) {
+    if (!ctx || !count || (n > 0 && !x)) return BB_ERR_NULL;
+    if (dtype != BB_F64 && dtype != BB_F32) return bb_fail(ctx, BB_ERR_ARG, "dtype %d", dtype);
+    if (n < 0) return bb_fail(ctx, BB_ERR_ARG, "n=%lld < 0", (long long)n);
+    *count = 0;
+    if (!n) return BB_OK;
+    if (!ctx->counter) BB_HIP(ctx, hipMalloc(&ctx->counter, sizeof(unsigned long long)));
+    int rc = bb_fill_zero(ctx, ctx->counter, sizeof(unsigned long long));
+    if (rc) return rc;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8 * 1024) blocks = 8 * 1024;
+    if (dtype == BB_F64)
+        k_count_nonfinite<double><<<(unsigned)blocks, 256, 0, ctx->stream>>>((const double *)x, n, ctx->counter);
+    else
+        k_count_nonfinite<float><<<(unsigned)blocks, 256, 0, ctx->stream>>>((const float *)x, n, ctx->counter);
+    BB_LAUNCH_CHECK(ctx);
+    unsigned long long c = 0;
+    BB_HIP(ctx, hipMemcpyAsync(&c, ctx->counter, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+    BB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *count = (int64_t)c;
+    return BB_OK;
+}
+
 extern "C" {
+
+int bb_count_nonfinite(bb_ctx *ctx, int dtype, const void *x, int64_t n, int64_t *count) {
+    const int rc = bb_nonfinite_device(ctx, dtype, x, n, count);
+    if (rc) return rc;
+    if (*count) return bb_fail(ctx, BB_ERR_NONFINITE, "%lld of %lld values are NaN or Inf", (long long)*count, (long long)n);
+    return BB_OK;
+}
 
 int bb_stream_copy(bb_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx || (bytes && (!dst || !src))) return BB_ERR_NULL;

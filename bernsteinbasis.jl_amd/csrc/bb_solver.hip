@@ -332,6 +332,8 @@ int bb_solver_destroy(bb_solver *s) {
 
 int bb_solver_set_state(bb_solver *s, const double *q_host) {
     if (!s || !q_host) return BB_ERR_NULL;
+    int rc = check_finite_host(s->ctx, "state", q_host, (int64_t)state_words(s));
+    if (rc) return rc;
     BB_HIP(s->ctx, hipMemcpyAsync(s->q[s->cur], q_host, state_words(s) * sizeof(double), hipMemcpyHostToDevice,
                                   s->ctx->stream));
     BB_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
@@ -349,6 +351,14 @@ int bb_solver_get_state(bb_solver *s, double *q_host) {
 int bb_solver_state_ptr(bb_solver *s, double **q_dev) {
     if (!s || !q_dev) return BB_ERR_NULL;
     *q_dev = s->q[s->cur];
+    return BB_OK;
+}
+
+int bb_solver_check_state(bb_solver *s, int64_t *count) {
+    if (!s || !count) return BB_ERR_NULL;
+    const int rc = bb_nonfinite_device(s->ctx, BB_F64, s->q[s->cur], (int64_t)state_words(s), count);
+    if (rc) return rc;
+    if (*count) return bb_fail(s->ctx, BB_ERR_NONFINITE, "the state holds %lld NaN or Inf values", (long long)*count);
     return BB_OK;
 }
 

@@ -45,20 +45,18 @@ struct StageArgs {
     int flux_kind;
     double sign[6];
     // elements of mixed degree (k_stage with MD = true): the neighbour across face f of
-    // element e has its state at q_nb + nboff[4 e + f] and its degree in bits 8..11 of
-    // conn.y; traces of another degree are brought to this degree by the matrices
-    // tr + troff[11 N + Nn] (nfp(N) x nfp(Nn), row-major); element e steps by dt * dte[e]
+    // element e has its state at q_nb + nboff[4 e + f] and its degree Nn in bits 8..11 of
+    // conn.y; element e steps by dt * dte[e].  A trace of another degree is brought to this
+    // degree by a transfer matrix: for Nn > N the dense projection at tr + conn.x
+    // (nfp(N) x nfp(Nn), row-major; trf in FP32), for Nn < N the sparse elevation whose
+    // row m holds entries erow[conn.x + m] .. erow[conn.x + m + 1] - 1 of ecol (neighbour
+    // face node) and evd / evf (weight)
     const double *q_nb;
     const int64_t *nboff;
     const double *tr;
-    const int64_t *troff;
     const double *dte;
-    // the same matrices in FP32 (mixed precision), and the elevations (Nn < N) as sparse
-    // rows: row m of pair (N, Nn) holds entries erow[eoff[11 N + Nn] + m] ..
-    // erow[eoff[11 N + Nn] + m + 1] - 1 of ecol (neighbour face node) and evd / evf (weight)
     const float *trf;
     const int32_t *erow;
-    const int64_t *eoff;
     const uint8_t *ecol;
     const double *evd;
     const float *evf;
@@ -72,6 +70,21 @@ constexpr double kLsrkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0
                               2277821191437.0 / 14882151754819.0};
 
 constexpr int kTraceMax = 66;  // face nodes of the highest degree of a fused kernel (N = 10)
+
+// v[d][f][m]: volume index of node m (linear order of the face triangle) of face f of a
+// degree-d element, d = 1..10 -- the face traces of neighbours of any degree, as the
+// mixed-degree kernels stage them (a table read in place of decoding (a,b) from m and
+// composing the volume index at a run-time degree)
+struct FaceVidAll {
+    uint16_t v[11][4][kTraceMax];
+    constexpr FaceVidAll() : v{} {
+        for (int d = 1; d <= 10; ++d)
+            for (int f = 0; f < 4; ++f)
+                for (int b = 0; b <= d; ++b)
+                    for (int a = 0; a <= d - b; ++a) v[d][f][lin2(d, a, b)] = (uint16_t)face_vidx(d, f, a, b);
+    }
+};
+__device__ constexpr FaceVidAll kFaceVidAll = FaceVidAll();
 
 // Lift organisation inside the stage kernel:
 //  GATHER   : every chain vector of every face is kept in LDS (slab s of face f at the
@@ -479,8 +492,69 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             }
         }
     }
-    if constexpr (MD) {
-        // whole face traces of neighbours of another degree -> s3[(element, face)][node][field]
+    if constexpr (MD && EB <= 2) {
+        // whole face traces of neighbours of another degree -> s3[(element, face)][node][field].
+        // Face c = (element, face) of the block stages nfp(Nn) records when its neighbour is
+        // an interior element of degree Nn != N, none otherwise; the staged records of the
+        // block's faces form one list, dealt to the threads in order.  The connectivity of
+        // the block's faces is read once (uniform: scalar loads), and every neighbour load of
+        // a chunk of up to SCH items per thread is issued before the chunk's first LDS store,
+        // so that the loads' latencies overlap instead of following one another.
+        constexpr int NF = EB * 4;
+        int cy[NF], cend[NF];
+        int64_t nbo[NF];
+        int tot = 0;
+#pragma unroll
+        for (int c = 0; c < NF; ++c) {
+            const bool ok = (c >> 2) < ne;
+            const int64_t fc = (e0 + (ok ? (c >> 2) : 0)) * 4 + (c & 3);
+            cy[c] = A.conn[fc].y;
+            nbo[c] = A.nboff[fc];
+            const int Nn = (cy[c] >> 8) & 15;
+            tot += (ok && ((cy[c] >> 5) & 7) == 0 && Nn != N) ? nfp2(Nn) : 0;
+            cend[c] = tot;
+        }
+        constexpr int MAXI = (NF * kTraceMax + NT - 1) / NT;  // items per thread at most
+        constexpr int SCH = MAXI < 3 ? MAXI : 3;
+#pragma unroll 1
+        for (int i0 = 0; i0 < MAXI && i0 * NT < tot; i0 += SCH) {
+            double v[SCH][F];
+            int dst[SCH];
+#pragma unroll
+            for (int u = 0; u < SCH; ++u) {
+                const int t = tid + (i0 + u) * NT;
+                const bool ok = t < tot;
+                // the item's face: the first c with t < cend[c] (selects, no indexed registers)
+                int yc = cy[0], beg = 0;
+                int64_t oc = nbo[0];
+                int c = 0;
+#pragma unroll
+                for (int cc = 1; cc < NF; ++cc) {
+                    const bool past = t >= cend[cc - 1];
+                    yc = past ? cy[cc] : yc;
+                    oc = past ? nbo[cc] : oc;
+                    beg = past ? cend[cc - 1] : beg;
+                    c += past;
+                }
+                const int Nn = ok ? (yc >> 8) & 15 : 1, jn = ok ? t - beg : 0;
+                const int npn = ok ? np3(Nn) : 1;
+                // items past the list read the first values of the state (never stored)
+                const double *qn = A.q_nb + (ok ? oc + kFaceVidAll.v[Nn][yc & 3][jn] : 0);
+#pragma unroll
+                for (int f = 0; f < F; ++f) v[u][f] = qn[f * npn];
+                dst[u] = ok ? c * kTraceMax + jn : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < SCH; ++u)
+                if (dst[u] >= 0) {
+                    T w[F];
+#pragma unroll
+                    for (int f = 0; f < F; ++f) w[f] = T(v[u][f]);
+                    recv_store<F, RS>(s3 + __mul24(dst[u], RS), w);
+                }
+        }
+    } else if constexpr (MD) {
+        // the same, item by item (blocks of four elements)
         for (int t = tid; t < ne * 4 * kTraceMax; t += NT) {
             const int e2 = t / (4 * kTraceMax), phi = (t / kTraceMax) & 3, jn = t % kTraceMax;
             const int2 cn = A.conn[(e0 + e2) * 4 + phi];
@@ -546,7 +620,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                     // unroll depths of these loops set the mixed-degree kernels' register
                     // peak: in FP32, 2 and 4 (instead of 4 and 8) leave 71-81 VGPRs instead of
                     // 82-103 and made the mixed batch 7% faster; FP64 keeps 4 and 8
-                    const int32_t *rp = A.erow + A.eoff[11 * N + Nn] + mp;
+                    const int32_t *rp = A.erow + cnv[r].x + mp;  // conn.x: the pair's first row
                     const int j1 = rp[1];
 #pragma unroll MIXED ? 2 : 4
                     for (int jj = rp[0]; jj < j1; ++jj) {
@@ -556,15 +630,36 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                         for (int f = 0; f < F; ++f) up[f] = fma(w, tv[f], up[f]);
                     }
                 } else {
-                    // L2 projection from a higher degree: a dense row
-                    const int64_t ro = A.troff[11 * N + Nn] + (int64_t)mp * nfn;
-#pragma unroll MIXED ? 4 : 8
-                    for (int jn = 0; jn < nfn; ++jn) {
-                        const T w = MIXED ? T(A.trf[ro + jn]) : T(A.tr[ro + jn]);
-                        recv_load<F, RS>(tn + __mul24(jn, RS), tv);
+                    // L2 projection from a higher degree: a dense row (conn.x: where the pair's
+                    // matrix starts), four entries at a time with the next four loaded ahead
+                    // (the tables end in kTransferPad zeros, so the read-ahead stays inside)
+                    const T *wr = (const T *)(MIXED ? (const void *)A.trf : (const void *)A.tr) + cnv[r].x + mp * nfn;
+                    T wn[4];
 #pragma unroll
-                        for (int f = 0; f < F; ++f) up[f] = fma(w, tv[f], up[f]);
+                    for (int u = 0; u < 4; ++u) wn[u] = wr[u];
+                    int jn = 0;
+#pragma unroll 1
+                    for (; jn + 4 <= nfn; jn += 4) {
+                        T wc[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            wc[u] = wn[u];
+                            wn[u] = wr[jn + 4 + u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            recv_load<F, RS>(tn + __mul24(jn + u, RS), tv);
+#pragma unroll
+                            for (int f = 0; f < F; ++f) up[f] = fma(wc[u], tv[f], up[f]);
+                        }
                     }
+#pragma unroll
+                    for (int u = 0; u < 3; ++u)
+                        if (jn + u < nfn) {
+                            recv_load<F, RS>(tn + __mul24(jn + u, RS), tv);
+#pragma unroll
+                            for (int f = 0; f < F; ++f) up[f] = fma(wn[u], tv[f], up[f]);
+                        }
                 }
 #pragma unroll
                 for (int f = 0; f < F; ++f) d[f] = up[f] - um[f];

@@ -56,6 +56,7 @@ const BB_LIFT_PLAIN = Cint(0); const BB_LIFT_ACC64 = Cint(1); const BB_LIFT_TWOS
 const BB_LIFT_ACC64_MFMA = Cint(3)
 const BB_PREC_F64 = Cint(0); const BB_PREC_MIXED = Cint(1)
 const BB_PATH_PLAIN = Cint(0); const BB_PATH_FUSED = Cint(1)
+const BB_ERR_NONFINITE = Cint(7)
 
 """`bb_layout`: strides in array elements of the element, node and field indices."""
 struct Layout
@@ -143,6 +144,16 @@ fill_zero!(dst::Ptr{Cvoid}, n::Integer, c::BBContext = context()) =
     check(ccall((:bb_fill_zero, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t), c.ptr, dst, Csize_t(n)), c)
 stream_copy!(dst::Ptr{Cvoid}, src::Ptr{Cvoid}, n::Integer, c::BBContext = context()) =
     check(ccall((:bb_stream_copy, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Csize_t), c.ptr, dst, src, Csize_t(n)), c)
+
+"""`count_nonfinite(x, n, T)`: the number of NaN / Inf values among n values of type T
+(Float64 or Float32) at device pointer x; 0 when all are finite."""
+function count_nonfinite(x::Ptr, n::Integer, ::Type{T} = Float64, c::BBContext = context()) where {T}
+    cnt = Ref{Int64}(0)
+    rc = ccall((:bb_count_nonfinite, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Int64, Ref{Int64}),
+               c.ptr, T === Float32 ? BB_F32 : BB_F64, x, Int64(n), cnt)
+    rc == BB_OK || rc == BB_ERR_NONFINITE || check(rc, c)
+    Int(cnt[])
+end
 
 timer_start(c::BBContext = context()) = check(ccall((:bb_timer_start, LIB), Cint, (Ptr{Cvoid},), c.ptr), c)
 function timer_stop(c::BBContext = context())
@@ -415,6 +426,14 @@ function state_ptr(s::BBSolver)
     p[]
 end
 
+"""`check_state(s)`: the number of NaN / Inf values in the solver's device state."""
+function check_state(s::BBSolver)
+    cnt = Ref{Int64}(0)
+    rc = ccall((:bb_solver_check_state, LIB), Cint, (Ptr{Cvoid}, Ref{Int64}), s.ptr, cnt)
+    rc == BB_OK || rc == BB_ERR_NONFINITE || check(rc, s.ctx)
+    Int(cnt[])
+end
+
 """`rhs!(du, s; path, prec)`: du = rhs_matvec!(q) of the current state (host array)."""
 rhs!(du::Array{Float64}, s::BBSolver; path::Symbol = :fused, prec::Symbol = :f64) =
     (check(ccall((:bb_solver_rhs, LIB), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Cdouble}), s.ptr, path_code(path),
@@ -524,6 +543,12 @@ function get_state(b::BBBatch)
     q = zeros(Float64, state_words(b))
     check(ccall((:bb_batch_get_state, LIB), Cint, (Ptr{Cvoid}, Ptr{Cdouble}), b.ptr, q), b.ctx)
     q
+end
+function check_state(b::BBBatch)
+    cnt = Ref{Int64}(0)
+    rc = ccall((:bb_batch_check_state, LIB), Cint, (Ptr{Cvoid}, Ref{Int64}), b.ptr, cnt)
+    rc == BB_OK || rc == BB_ERR_NONFINITE || check(rc, b.ctx)
+    Int(cnt[])
 end
 rhs!(du::Vector{Float64}, b::BBBatch; prec::Symbol = :f64) =
     (check(ccall((:bb_batch_rhs, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Cdouble}), b.ptr, prec_code(prec), du), b.ctx); du)

@@ -37,7 +37,8 @@ enum bb_status {
     BB_ERR_HIP = 3,         /* HIP runtime error (message in bb_last_error) */
     BB_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
     BB_ERR_STATE = 5,       /* call out of order (e.g. graph launch before capture) */
-    BB_ERR_NOMEM = 6        /* host memory allocation (or other host-side) failure */
+    BB_ERR_NOMEM = 6,       /* host memory allocation (or other host-side) failure */
+    BB_ERR_NONFINITE = 7    /* NaN or Inf in an input array or in a solver state */
 };
 
 enum bb_dtype { BB_F64 = 0, BB_F32 = 1 };
@@ -91,6 +92,12 @@ int bb_fill_zero(bb_ctx *ctx, void *dst, size_t bytes); /* kernel fill, capturab
  * counterpart. */
 int bb_stream_copy(bb_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bb_synchronize(bb_ctx *ctx);
+/* Counts the NaN and Inf values among n values of dtype at the device pointer x (waits for
+ * the context's stream).  Returns BB_ERR_NONFINITE when the count is non-zero (the count is
+ * written either way).  The operator entry points do not scan their inputs (the reference
+ * checks nothing on its hot path, @inbounds); callers that need it scan with this.  No
+ * reference counterpart. */
+int bb_count_nonfinite(bb_ctx *ctx, int dtype, const void *x, int64_t n, int64_t *count);
 
 /* HIP-event timing on the context stream.  bb_timer_stop returns the milliseconds
  * between the matching start and stop. */
@@ -171,7 +178,8 @@ int bb_interface_flux(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, con
  *               perm [K][4] (0..5: neighbour face vertex m = my face vertex p[m], p in
  *               lexicographic order of the permutations of {0,1,2}), bc [K][4]
  *               (0 interior, 1 wall), bc_sign [F] (uP = bc_sign .* uM on walls).
- * State layout: [K][F][Np] FP64. */
+ * State layout: [K][F][Np] FP64.  Non-finite mesh values (geo, J, nJ) are refused with
+ * BB_ERR_NONFINITE, as is a state with a NaN or Inf in bb_solver_set_state. */
 int bb_solver_create(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t K, const double *geo_host,
                      const double *J_host, const double *nJ_host, const int32_t *nbr_host,
                      const int8_t *nbr_face_host, const int8_t *perm_host, const int8_t *bc_host,
@@ -181,6 +189,9 @@ int bb_solver_set_state(bb_solver *s, const double *q_host);
 int bb_solver_get_state(bb_solver *s, double *q_host);
 /* device pointer of the current state ([K][F][Np] FP64) */
 int bb_solver_state_ptr(bb_solver *s, double **q_dev);
+/* count of NaN / Inf values in the current device state (a diverged run); BB_ERR_NONFINITE
+ * when it is non-zero.  No reference counterpart. */
+int bb_solver_check_state(bb_solver *s, int64_t *count);
 /* du = rhs(q) of the current state, copied to du_host ([K][F][Np] FP64).  For
  * BB_PREC_MIXED the values are the FP32 residual register widened to FP64. */
 int bb_solver_rhs(bb_solver *s, int path, int prec, double *du_host);
@@ -255,6 +266,9 @@ int bb_batch_destroy(bb_batch *b);
 int bb_batch_state_words(bb_batch *b, int64_t *words);
 int bb_batch_set_state(bb_batch *b, const double *q_host);
 int bb_batch_get_state(bb_batch *b, double *q_host);
+/* as bb_solver_check_state, over every degree group (non-finite mesh values, steps, transfer
+ * entries and states are refused as for a solver) */
+int bb_batch_check_state(bb_batch *b, int64_t *count);
 int bb_batch_rhs(bb_batch *b, int prec, double *du_host);
 int bb_batch_step(bb_batch *b, int prec, double dt_scale, int nsteps);
 int bb_batch_graph_capture(bb_batch *b, int prec, double dt_scale, int nsteps);

@@ -47,6 +47,19 @@ static void check_tables() {
     }
 }
 
+// the mixed-degree kernels' table of every degree's face-node volume indices against the
+// run-time decode it replaces
+static void check_face_table() {
+    static const FaceVidAll t;
+    for (int d = 1; d <= kStageMaxN; ++d)
+        for (int f = 0; f < 4; ++f)
+            for (int m = 0; m < nfp2(d); ++m) {
+                int a, b;
+                decode2(d, m, a, b);
+                CHECK(t.v[d][f][m] == face_vidx(d, f, a, b) && t.v[d][f][m] < np3(d));
+            }
+}
+
 static void check_solver_setup() {
     bb_ctx ctx;
     const int N = 3;
@@ -90,7 +103,12 @@ static void check_solver_setup() {
         CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bad_bc, J, nJ) == BB_ERR_ARG);
         double J0[1] = {0.0}, Jnan[1] = {NAN}, nJ0[12] = {0, -1, 0, 0, 0, 0, -1, 0, 0, 0, 0, -1};
         CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bc, J0, nJ) == BB_ERR_ARG);
-        CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bc, Jnan, nJ) == BB_ERR_ARG);
+        CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bc, Jnan, nJ) == BB_ERR_NONFINITE);
+        double geo_inf[9] = {1, 0, 0, 0, INFINITY, 0, 0, 0, 1}, nJ_nan[12] = {0, -1, 0, 1, 1, NAN, -1, 0, 0, 0, 0, -1};
+        CHECK(run(0, 1, N, K, geo_inf, nbr, nf, perm, bc, J, nJ) == BB_ERR_NONFINITE);
+        CHECK(std::string(ctx.err).find("geo[4]") != std::string::npos);
+        CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bc, J, nJ_nan) == BB_ERR_NONFINITE);
+        CHECK(std::string(ctx.err).find("nJ[5]") != std::string::npos);
         CHECK(run(0, 1, N, K, geo, nbr, nf, perm, bc, J, nJ0) == BB_ERR_ARG);
         CHECK(std::string(ctx.err).find("zero face normal") != std::string::npos);
     }
@@ -114,11 +132,11 @@ static void check_batch_setup() {
     const int n3 = nfp2(3), n4 = nfp2(4);
     std::vector<double> tr((size_t)2 * n3 * n4, 0.0);
     std::vector<int64_t> toff(121, -1);
-    toff[11 * 3 + 4] = 0;
-    toff[11 * 4 + 3] = n3 * n4;
+    toff[11 * 4 + 3] = 0;        // elevation 3 -> 4 first
+    toff[11 * 3 + 4] = n3 * n4;  // then the projection 4 -> 3
     for (int r = 0; r < n3; ++r)
-        for (int c = 0; c < n4; ++c) tr[(size_t)r * n4 + c] = 1.0 / n4;
-    for (int r = 0; r < n4; ++r) tr[(size_t)n3 * n4 + (size_t)r * n3 + r % n3] = 1.0;  // one entry per row
+        for (int c = 0; c < n4; ++c) tr[(size_t)n3 * n4 + (size_t)r * n4 + c] = 1.0 / n4;
+    for (int r = 0; r < n4; ++r) tr[(size_t)r * n3 + r % n3] = 1.0;  // one entry per row
     const int64_t tl = (int64_t)tr.size();
     auto run = [&](int pde, int64_t k, const int32_t *d, const int64_t *to, int64_t len) {
         BatchHost h;
@@ -132,14 +150,18 @@ static void check_batch_setup() {
         CHECK(h.groups.size() == 2 && h.groups[1].qoff == 4 * np3(3) && h.words == 4 * (np3(3) + np3(4)));
         CHECK(h.nboff[0] == 4 * np3(3) && h.nboff[4] == 0 && (h.conn[0].y >> 8) == 4 && (h.conn[4].y >> 8) == 3);
         CHECK(h.eoff[11 * 4 + 3] == 0 && h.erow.size() == (size_t)n4 + 1 && h.evd.size() == (size_t)n4);
-        CHECK(h.trf.size() == tr.size() && h.evf.size() == h.evd.size());
+        CHECK(h.trf.size() == tr.size() + kTransferPad && h.evf.size() == h.evd.size());
+        CHECK(h.trf[tr.size()] == 0.f && h.trf[tr.size() + kTransferPad - 1] == 0.f);
+        // conn.x: where each face's transfer rows start (projection for element 0, whose
+        // neighbour has the higher degree; the elevation rows for element 1; 0 on walls)
+        CHECK(h.conn[0].x == n3 * n4 && h.conn[4].x == h.eoff[11 * 4 + 3] && h.conn[1].x == 0);
     }
     CHECK(run(BB_PDE_MAXWELL, K, deg, toff.data(), tl) == BB_ERR_UNSUPPORTED);
     CHECK(run(BB_PDE_WAVE, 0, deg, toff.data(), tl) == BB_ERR_ARG);
     CHECK(run(BB_PDE_WAVE, kMaxElements + 1, deg, toff.data(), tl) == BB_ERR_ARG);
     CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), -1) == BB_ERR_ARG);
     CHECK(std::string(ctx.err).find("transfer_len") != std::string::npos);
-    CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), tl - 1) == BB_ERR_ARG);  // the 4 <- 3 table is cut short
+    CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), tl - 1) == BB_ERR_ARG);  // the 3 <- 4 table is cut short
     CHECK(run(BB_PDE_WAVE, K, deg, nullptr, tl) == BB_ERR_NULL);
     int32_t down[2] = {4, 3}, high[2] = {3, kStageMaxN + 1};
     CHECK(run(BB_PDE_WAVE, K, down, toff.data(), tl) == BB_ERR_ARG);
@@ -148,6 +170,36 @@ static void check_batch_setup() {
     missing[11 * 3 + 4] = -1;
     CHECK(run(BB_PDE_WAVE, K, deg, missing.data(), tl) == BB_ERR_ARG);
     CHECK(std::string(ctx.err).find("no transfer table from degree 4 to 3") != std::string::npos);
+    // non-finite steps, transfer entries and geometry
+    dt[1] = NAN;
+    CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), tl) == BB_ERR_NONFINITE);
+    CHECK(std::string(ctx.err).find("dt[1]") != std::string::npos);
+    dt[1] = 1e-3;
+    tr[7] = -INFINITY;
+    CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), tl) == BB_ERR_NONFINITE);
+    CHECK(std::string(ctx.err).find("transfer[7]") != std::string::npos);
+    tr[7] = 0.0;
+    geo[17] = NAN;
+    CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), tl) == BB_ERR_NONFINITE);
+    geo[17] = 0.0;
+    CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), tl) == BB_OK);
+}
+
+static void check_nonfinite_scan() {
+    bb_ctx ctx;
+    std::vector<double> x(1000, 1.0);
+    CHECK(first_nonfinite(x.data(), (int64_t)x.size()) == -1);
+    CHECK(first_nonfinite(x.data(), 0) == -1);
+    x[999] = INFINITY;
+    CHECK(first_nonfinite(x.data(), (int64_t)x.size()) == 999);
+    CHECK(first_nonfinite(x.data(), 999) == -1);
+    x[3] = -NAN;
+    CHECK(first_nonfinite(x.data(), (int64_t)x.size()) == 3);
+    x[3] = 1e308 * 10.0;  // overflows to Inf
+    CHECK(check_finite_host(&ctx, "state", x.data(), (int64_t)x.size()) == BB_ERR_NONFINITE);
+    CHECK(std::string(ctx.err).find("state[3]") != std::string::npos);
+    x[3] = x[999] = 1.0;
+    CHECK(check_finite_host(&ctx, "state", x.data(), (int64_t)x.size()) == BB_OK);
 }
 
 static void check_guard_and_errors() {
@@ -175,9 +227,11 @@ static void check_guard_and_errors() {
 
 int main() {
     check_tables();
+    check_face_table();
     check_solver_setup();
     check_batch_setup();
     check_guard_and_errors();
+    check_nonfinite_scan();
     if (g_fail) {
         fprintf(stderr, "%d of %d host checks failed\n", g_fail, g_checks);
         return 1;
