@@ -430,7 +430,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             const bool zero = A.first || A.rhs_only;
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                const TR x = res[(eg * F + f) * NP + n];
+                const TR x = __builtin_nontemporal_load(&res[(eg * F + f) * NP + n]);
                 rv[f] = zero ? TR(0) : x;
             }
         }
@@ -442,7 +442,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 #pragma unroll
             for (int f = 0; f < F; ++f) {
                 qv[f] = A.q_in[(eg * F + f) * NP + n];
-                rv[f] = (A.first || A.rhs_only) ? TR(0) : res[(eg * F + f) * NP + n];
+                rv[f] = (A.first || A.rhs_only) ? TR(0) : __builtin_nontemporal_load(&res[(eg * F + f) * NP + n]);
             }
             invJ = A.invJ[eg];
             dte = MD ? A.dt * A.dte[eg] : A.dt;
@@ -1004,8 +1004,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             } else {
                 const double r = A.first ? dte * rhs : fma(A.ca, (double)rv[f], dte * rhs);
                 const TR rs = TR(r);
-                res[g] = rs;
-                A.q_out[g] = qv[f] + A.cb * (double)rs;
+                __builtin_nontemporal_store(rs, &res[g]);
+                __builtin_nontemporal_store(qv[f] + A.cb * (double)rs, &A.q_out[g]);
             }
         }
     }
