@@ -16,7 +16,12 @@ errors     : per-field relative L2 / max error of the mixed path against the fus
 roofline   : the fused mixed stage kernel, HBM-bound; achieved = algorithmic bytes per
              stage / average stage duration (HIP events on the library's stream).
 cpu_baseline: the oracle's OpenMP restatement of the reference RHS + LSRK45 (FP64) on the
-             host cores, on a bounded sample of the same workload.
+             host cores, on a bounded sample of the same workload (16^3 x 6 elements); on
+             that sample the GPU's fused mixed and FP64 states after the same steps are
+             compared with the oracle's, per field (gpu_vs_oracle).
+also       : the operator microbenchmark (N=7, K=262144), the 3-D wave at N=7, the dense
+             nodal comparator, the batched mixed-degree workload and the single-face lift
+             conditioning study (N = 8, 11, 14, 17, K=65536), each with its own sizes.
 """
 from __future__ import annotations
 
@@ -461,6 +466,7 @@ def cpu_baseline(n_sample, steps):
     ma = O.MeshArrays(m, N, [1, 1, 1, -1, -1, -1])
     q = np.ascontiguousarray(basis.interpolate(m, N, maxwell_ic))
     dt = 0.5 * m.h / N ** 2
+    q0 = q.copy()
     O.lsrk_step(1, N, q, dt, ma, nthreads=threads, L=L)  # warm
     t = time.perf_counter()
     for _ in range(steps):
@@ -472,11 +478,26 @@ def cpu_baseline(n_sample, steps):
         O.rhs(1, N, q, ma, nthreads=threads, L=L)
     el_rhs = (time.perf_counter() - t) / reps
     dofs = m.K * 220 * 6
+    # the oracle as the checker: the GPU's fused mixed and FP64 paths after the same steps
+    # from the same initial state on this mesh, per field against the oracle's state
+    import bbdg
+
+    names = ["Hx", "Hy", "Hz", "Ex", "Ey", "Ez"]
+    vs_oracle = {"steps": steps + 1, "K": m.K, "reference": "oracle C restatement (FP64) of the reference RHS + "
+                                                            "LSRK45, same mesh, initial state and dt"}
+    s = bbdg.Solver(m, N, "maxwell", flux="upwind")
+    for prec in ("mixed", "f64"):
+        s.set_state(q0)
+        s.step(dt, steps + 1, "fused", prec)
+        per_field, tot = field_errors(s.get_state(), q, names)
+        vs_oracle[prec] = {"rel_l2_all": tot, "fields": per_field}
+        log(f"gpu fused {prec} vs oracle after {steps + 1} steps (K={m.K}): rel L2 {tot:.3e}")
+    s.close()
     return {"value": dofs * steps / el, "unit": "DOF-updates/s per step", "cores": threads, "kind": "port",
             "per_rhs_value": dofs / el_rhs, "ms_per_step": el / steps * 1e3,
             "nproc": os.cpu_count(), "affinity_cores": avail, "omp_num_threads_env": cap,
             "cpu_model": O.cpu_model(), "flags": " ".join(O.BASELINE_FLAGS + ["-fopenmp"]),
-            "schedule": "static", "K": m.K, "N": N, "build_s": t_build,
+            "schedule": "static", "K": m.K, "N": N, "build_s": t_build, "gpu_vs_oracle": vs_oracle,
             "sample": f"{steps} LSRK45 steps (+{reps} RHS evaluations) of Maxwell N=9 FP64 on the {n_sample}^3x6 "
                       f"(K={m.K}) jittered PEC mesh: the oracle's C restatement of the reference's threaded RHS, "
                       f"gcc {' '.join(O.BASELINE_FLAGS)} on this host, OpenMP static schedule, {threads} threads "
@@ -496,6 +517,7 @@ def main():
     ap.add_argument("--no-ops", action="store_true", help="skip the operator microbenchmark")
     ap.add_argument("--lift-study", action="store_true",
                     help="run only the lift conditioning study (single-face lift precision variants)")
+    ap.add_argument("--no-lift-study", action="store_true", help="skip the lift conditioning study")
     ap.add_argument("--ops-only", action="store_true", help="run only the operator microbenchmark")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-degree workload")
     ap.add_argument("--no-wave7", action="store_true", help="skip the N=7 wave workload")
@@ -606,6 +628,7 @@ def main():
     batch = None
     if not args.no_batch:
         batch = batch_bench(bbdg.default_context(), steps=args.batch_steps)
+    lift_cond = None if args.no_lift_study else lift_study(bbdg.default_context())
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_n, args.cpu_steps)
@@ -649,6 +672,7 @@ def main():
         "wave_n7": wave7,
         "dense_nodal_comparator": nodal,
         "batch_mixed_degree": batch,
+        "lift_conditioning": lift_cond,
     }
     print(json.dumps(out))
 

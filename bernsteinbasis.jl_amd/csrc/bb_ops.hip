@@ -487,10 +487,26 @@ __global__ void k_plain_combine(bool plus, int FNP, double *__restrict__ du, Ptr
     }
 }
 
+// res = a res + dt du (res not read in the first stage), q = q + b res.  Pairs of values
+// per thread as 16-byte accesses (the arrays are allocations of the library, 16-byte
+// aligned), the odd last value by thread 0; du and res are read once and res written
+// once, as non-temporal accesses.
 __global__ void k_lsrk_update(int64_t n, double *__restrict__ q, double *__restrict__ res, const double *__restrict__ du,
                               double a, double b, double dt, int first) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        double r = first ? dt * du[t] : a * res[t] + dt * du[t];
+    typedef double V2 __attribute__((ext_vector_type(2)));
+    const int64_t n2 = n >> 1;
+    V2 *q2 = reinterpret_cast<V2 *>(q), *r2 = reinterpret_cast<V2 *>(res);
+    const V2 *d2 = reinterpret_cast<const V2 *>(du);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (int64_t)gridDim.x * blockDim.x) {
+        const V2 d = __builtin_nontemporal_load(&d2[t]);
+        V2 r = dt * d;
+        if (!first) r = a * __builtin_nontemporal_load(&r2[t]) + r;
+        __builtin_nontemporal_store(r, &r2[t]);
+        q2[t] = q2[t] + b * r;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int64_t t = n - 1;
+        const double r = first ? dt * du[t] : a * res[t] + dt * du[t];
         res[t] = r;
         q[t] = q[t] + b * r;
     }
@@ -620,7 +636,9 @@ int bb_plain_combine(bb_ctx *ctx, int pde, int N, int64_t K, double *du, double 
 
 int bb_lsrk_update(bb_ctx *ctx, int64_t n, double *q, double *res, const double *du, double a, double b, double dt,
                    int first) {
-    k_lsrk_update<<<grid_cap(n), 256, 0, ctx->stream>>>(n, q, res, du, a, b, dt, first);
+    if ((((uintptr_t)q | (uintptr_t)res | (uintptr_t)du) & 15u) != 0)
+        return bb_fail(ctx, BB_ERR_ARG, "the RK update needs 16-byte aligned arrays");
+    k_lsrk_update<<<grid_cap((n + 1) / 2), 256, 0, ctx->stream>>>(n, q, res, du, a, b, dt, first);
     BB_LAUNCH_CHECK(ctx);
     return BB_OK;
 }
