@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kLiftR) void k_lift4r(const TI *__restrict__ x, TI 
 template <int N, int F, typename TI, typename TC> struct LiftXCfg {
     static constexpr int NP = np3(N), NFP = nfp2(N);
     static constexpr int W = LiftRCfg<N, F, TI, TC>::W;
-    static constexpr size_t lds = (size_t)4 * (NP + NFP) * F * sizeof(TC) + (N + 1) * 8;
+    static constexpr size_t lds = (size_t)4 * NP * F * sizeof(TC);  // the face records live in the slabs
     static constexpr bool valid = N >= 5 && NFP <= 64 && lds <= 64 * 1024;
 };
 
@@ -563,14 +563,18 @@ __device__ __forceinline__ void lift_chain_group(TC *sw, int PSS, int g, const L
 template <int N, int F, typename TI, typename TC, typename TA, int NW, bool L0M = false>
 __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI *__restrict__ out, int64_t K) {
     constexpr int NP = np3(N), NFP = nfp2(N), W = LiftXCfg<N, F, TI, TC>::W;
-    constexpr int PSS = 4 * NP * W, PSX = 4 * NFP * W;  // plane strides
+    constexpr int PSS = 4 * NP * W;  // plane stride
     constexpr int PA = LiftXPhases<N>::A, PB = LiftXPhases<N>::B;
     constexpr int FQ = 4 / NW;  // faces per wave until the chains are packed
     static_assert(NFP <= 64 && (NW == 2 || NW == 4), "one lane per face node, 2 or 4 waves");
     static_assert(!L0M || (std::is_same_v<TC, float> && F == 4), "matrix-core L0: FP32, 4 fields (16 columns)");
     const LiftTab<N> &tb = kLiftTab<N>;
-    __shared__ __align__(16) TC S[4 * NP * F];   // [plane][face][slab node][W fields]
-    __shared__ __align__(16) TC X[4 * NFP * F];  // [plane][face][face node][W fields]
+    __shared__ __align__(16) TC S[4 * NP * F];  // [plane][face][slab node][W fields]
+    // face f's data, record m, sit at S + (f NP + Nfp + m) W: in the face's own slab 1..
+    // records, which only its chain writes, and only after the face's L0 has read them
+    // (2 Nfp <= Np for N >= 3)
+    TC *const X = S + NFP * W;
+    static_assert(2 * NFP <= NP, "face records inside slabs 1..");
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid / 64);  // faces w, w + NW, ...
     const int64_t e = blockIdx.x;
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI
     }
 #pragma unroll
     for (int q = 0; q < FQ; ++q)
-        if (node) plane_store<F, W>(X + __mul24((w + q * NW) * NFP + lane, W), PSX, y[q]);
+        if (node) plane_store<F, W>(X + __mul24((w + q * NW) * NP + lane, W), PSS, y[q]);
     wave_lds_sync();
     // L0 (without its 1/2): the 7-point stencil of l0_prep_rec, same weights and order.
     // (Values of lanes outside a triangle are never read with a non-zero weight: a lane in
@@ -611,7 +615,7 @@ __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI
             const float *ap = lt.a + (mt * 64 + lane) * KT;
             for (int kk = lt.klo[mt]; kk <= lt.khi[mt]; ++kk) {
                 const int k = 4 * kk + (lane >> 4);
-                const float bv = k < NFP ? X[__mul24(fc * NFP + k, W) + fld] : 0.f;
+                const float bv = k < NFP ? X[__mul24(fc * NP + k, W) + fld] : 0.f;
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[kk], bv, acc, 0, 0, 0);
             }
 #pragma unroll
@@ -626,7 +630,7 @@ __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI
 #pragma unroll
     for (int q = 0; q < FQ; ++q) {
         const int fc = w + q * NW;
-        const TC *xw = X + fc * NFP * W;
+        const TC *xw = X + fc * NP * W;
         TC *sw = S + fc * NP * W;
         TC xs[F];
         if constexpr (L0M) {
@@ -635,10 +639,10 @@ __global__ __launch_bounds__(64 * NW) void k_lift4x(const TI *__restrict__ x, TI
             if constexpr (PA >= 1) plane_load<F, W>(sw + __mul24(lane, W), PSS, xs);
         } else {
             TC y1[F], y3[F], y4[F], y6[F];
-            plane_load<F, W>(xw + pt.o1, PSX, y1);
-            plane_load<F, W>(xw + pt.o3, PSX, y3);
-            plane_load<F, W>(xw + pt.o4, PSX, y4);
-            plane_load<F, W>(xw + pt.o6, PSX, y6);
+            plane_load<F, W>(xw + pt.o1, PSS, y1);
+            plane_load<F, W>(xw + pt.o3, PSS, y3);
+            plane_load<F, W>(xw + pt.o4, PSS, y4);
+            plane_load<F, W>(xw + pt.o6, PSS, y6);
 #pragma unroll
             for (int f = 0; f < F; ++f) {
                 TC v = pt.w0 * y[q][f];

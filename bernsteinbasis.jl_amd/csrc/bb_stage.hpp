@@ -232,6 +232,16 @@ inline std::vector<uint16_t> stage_tables(int N) {
     return t;
 }
 
+// the slab weights lw_chain(N, s), s = 0..N, as a table (one load per thread in the
+// prologue instead of a select chain over the constants)
+template <int N> struct LwTab {
+    double v[N + 1];
+    constexpr LwTab() : v{} {
+        for (int s = 0; s <= N; ++s) v[s] = lw_chain(N, s);
+    }
+};
+template <int N> __device__ constexpr LwTab<N> kLwTab = LwTab<N>();
+
 constexpr int kXcds = 8;
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
     const unsigned q = nb / kXcds, r = nb % kXcds;
@@ -287,11 +297,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
         pk = tab[n];
         if constexpr (Cfg::GATHER) gp = reinterpret_cast<const uint2 *>(tab + Cfg::TABG)[n];
     }
-    if (tid <= N) {
-#pragma unroll
-        for (int s = 0; s <= N; ++s)
-            if (tid == s) lw[s] = lw_chain(N, s);
-    }
+    if (tid <= N) lw[tid] = kLwTab<N>.v[tid];
     // matrix-core L0: this wave's first row tile of L0 as A operands, needed after the flux
     constexpr int MT = Cfg::MT, KT = Cfg::KT, NW = NT / 64;
     typedef float v4f __attribute__((ext_vector_type(4)));
@@ -1002,7 +1008,8 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             if (A.rhs_only) {
                 res[g] = TR(rhs);
             } else {
-                const double r = A.first ? dte * rhs : fma(A.ca, (double)rv[f], dte * rhs);
+                // in the first stage rv is 0 (not loaded), so this is exactly dte * rhs
+                const double r = fma(A.ca, (double)rv[f], dte * rhs);
                 const TR rs = TR(r);
                 __builtin_nontemporal_store(rs, &res[g]);
                 __builtin_nontemporal_store(qv[f] + A.cb * (double)rs, &A.q_out[g]);

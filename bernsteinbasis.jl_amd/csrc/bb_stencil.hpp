@@ -220,6 +220,68 @@ template <int F, int W, typename T> __device__ __forceinline__ void plane_store(
     }
 }
 
+// The 13 stencil points of node idx = (i,j,k) of a degree-N coefficient vector (record
+// indices, in the order grad2_pts reads them): the centre, the i-1, j-1 and k-1 neighbours,
+// then for d/dr the points (i+1,j-1), (i+1,k-1), (i+1); for d/ds (j+1,i-1), (j+1,k-1),
+// (j+1); for d/dt (k+1,i-1), (k+1,j-1), (k+1) -- each replaced by the centre where its
+// weight (i, j, k or l = N-i-j-k) is zero.  Shared by the kernels' run-time offsets and
+// the per-degree tables built on the host.
+__host__ __device__ constexpr void grad_points(int N, int idx, int i, int j, int k, int (&p)[13]) {
+    const int l = N - i - j - k;
+    const int rl = N - k - j + 1;
+    const int sk = (N - k + 1) * (N - k + 2) / 2, skm = (N - k + 2) * (N - k + 3) / 2;
+    const int km = idx + j - skm, kp = idx + sk - j;
+    p[0] = idx;
+    p[1] = i > 0 ? idx - 1 : idx;
+    p[2] = j > 0 ? idx - rl - 1 : idx;
+    p[3] = k > 0 ? km : idx;
+    p[4] = j > 0 ? idx - rl : idx;
+    p[5] = k > 0 ? km + 1 : idx;
+    p[6] = l > 0 ? idx + 1 : idx;
+    p[7] = i > 0 ? idx + rl - 1 : idx;
+    p[8] = k > 0 ? km + rl + 1 : idx;
+    p[9] = l > 0 ? idx + rl : idx;
+    p[10] = i > 0 ? kp - 1 : idx;
+    p[11] = j > 0 ? kp - rl : idx;
+    p[12] = l > 0 ? kp : idx;
+}
+
+// grad2_rec's arithmetic on given record offsets o[13] (in values, i.e. index times F) and
+// weights ti, tj, tk, tl: the four weighted differences per derivative in a fixed order
+template <int F, int G, typename T>
+__device__ __forceinline__ void grad2_pts(const T *x, const int (&o)[13], T ti, T tj, T tk, T tl, T (&dr)[G],
+                                          T (&ds)[G], T (&dt)[G]) {
+    T xc[G], xim[G], xjm[G], xkm[G], y[G], z[G];
+    recv_load<G, F>(x + o[0], xc);
+    recv_load<G, F>(x + o[1], xim);
+    recv_load<G, F>(x + o[2], xjm);
+    recv_load<G, F>(x + o[3], xkm);
+    // d/dr: i (x_c - x_{i-1}) + j (x_{i+1,j-1} - x_{j-1}) + k (x_{i+1,k-1} - x_{k-1}) + l (x_{i+1} - x_c)
+    recv_load<G, F>(x + o[4], y);
+    recv_load<G, F>(x + o[5], z);
+#pragma unroll
+    for (int f = 0; f < G; ++f) dr[f] = fma(tk, z[f] - xkm[f], fma(tj, y[f] - xjm[f], ti * (xc[f] - xim[f])));
+    recv_load<G, F>(x + o[6], y);
+#pragma unroll
+    for (int f = 0; f < G; ++f) dr[f] = fma(tl, y[f] - xc[f], dr[f]);
+    // d/ds: i (x_{i-1,j+1} - x_{i-1}) + j (x_c - x_{j-1}) + k (x_{j+1,k-1} - x_{k-1}) + l (x_{j+1} - x_c)
+    recv_load<G, F>(x + o[7], y);
+    recv_load<G, F>(x + o[8], z);
+#pragma unroll
+    for (int f = 0; f < G; ++f) ds[f] = fma(tk, z[f] - xkm[f], fma(tj, xc[f] - xjm[f], ti * (y[f] - xim[f])));
+    recv_load<G, F>(x + o[9], y);
+#pragma unroll
+    for (int f = 0; f < G; ++f) ds[f] = fma(tl, y[f] - xc[f], ds[f]);
+    // d/dt: i (x_{i-1,k+1} - x_{i-1}) + j (x_{j-1,k+1} - x_{j-1}) + k (x_c - x_{k-1}) + l (x_{k+1} - x_c)
+    recv_load<G, F>(x + o[10], y);
+    recv_load<G, F>(x + o[11], z);
+#pragma unroll
+    for (int f = 0; f < G; ++f) dt[f] = fma(tk, xc[f] - xkm[f], fma(tj, z[f] - xjm[f], ti * (y[f] - xim[f])));
+    recv_load<G, F>(x + o[12], y);
+#pragma unroll
+    for (int f = 0; f < G; ++f) dt[f] = fma(tl, y[f] - xc[f], dt[f]);
+}
+
 // Twice the three reference-coordinate derivatives of G fields of node records F values
 // apart (x + F m holds node m; x points at the group's first field): the 13 stencil
 // points of grad_ref_ld, each read once for the group, the four weighted differences per
@@ -229,42 +291,13 @@ template <int N, int F, int G, typename T>
 __device__ __forceinline__ void grad2_rec(const T *x, int idx, int i, int j, int k, T (&dr)[G], T (&ds)[G],
                                           T (&dt)[G]) {
     const int l = N - i - j - k;
-    const int rl = N - k - j + 1;
-    const int sk = (N - k + 1) * (N - k + 2) / 2;
-    const int skm = (N - k + 2) * (N - k + 3) / 2;
-    // record offsets (index times F) of the stencil points, 24-bit multiplies only
-    const int cF = __mul24(idx, F), rlF = __mul24(rl, F);
-    const int kmF = cF + __mul24(j - skm, F), kpF = cF + __mul24(sk - j, F);
-    const T ti = T(i), tj = T(j), tk = T(k), tl = T(l);
-    T xc[G], xim[G], xjm[G], xkm[G], y[G], z[G];
-    recv_load<G, F>(x + cF, xc);
-    recv_load<G, F>(x + (i > 0 ? cF - F : cF), xim);
-    recv_load<G, F>(x + (j > 0 ? cF - rlF - F : cF), xjm);
-    recv_load<G, F>(x + (k > 0 ? kmF : cF), xkm);
-    // d/dr: i (x_c - x_{i-1}) + j (x_{i+1,j-1} - x_{j-1}) + k (x_{i+1,k-1} - x_{k-1}) + l (x_{i+1} - x_c)
-    recv_load<G, F>(x + (j > 0 ? cF - rlF : cF), y);
-    recv_load<G, F>(x + (k > 0 ? kmF + F : cF), z);
+    int p[13];
+    grad_points(N, idx, i, j, k, p);
+    // record offsets (index times F), 24-bit multiplies only
+    int o[13];
 #pragma unroll
-    for (int f = 0; f < G; ++f) dr[f] = fma(tk, z[f] - xkm[f], fma(tj, y[f] - xjm[f], ti * (xc[f] - xim[f])));
-    recv_load<G, F>(x + (l > 0 ? cF + F : cF), y);
-#pragma unroll
-    for (int f = 0; f < G; ++f) dr[f] = fma(tl, y[f] - xc[f], dr[f]);
-    // d/ds: i (x_{i-1,j+1} - x_{i-1}) + j (x_c - x_{j-1}) + k (x_{j+1,k-1} - x_{k-1}) + l (x_{j+1} - x_c)
-    recv_load<G, F>(x + (i > 0 ? cF + rlF - F : cF), y);
-    recv_load<G, F>(x + (k > 0 ? kmF + rlF + F : cF), z);
-#pragma unroll
-    for (int f = 0; f < G; ++f) ds[f] = fma(tk, z[f] - xkm[f], fma(tj, xc[f] - xjm[f], ti * (y[f] - xim[f])));
-    recv_load<G, F>(x + (l > 0 ? cF + rlF : cF), y);
-#pragma unroll
-    for (int f = 0; f < G; ++f) ds[f] = fma(tl, y[f] - xc[f], ds[f]);
-    // d/dt: i (x_{i-1,k+1} - x_{i-1}) + j (x_{j-1,k+1} - x_{j-1}) + k (x_c - x_{k-1}) + l (x_{k+1} - x_c)
-    recv_load<G, F>(x + (i > 0 ? kpF - F : cF), y);
-    recv_load<G, F>(x + (j > 0 ? kpF - rlF : cF), z);
-#pragma unroll
-    for (int f = 0; f < G; ++f) dt[f] = fma(tk, xc[f] - xkm[f], fma(tj, z[f] - xjm[f], ti * (y[f] - xim[f])));
-    recv_load<G, F>(x + (l > 0 ? kpF : cF), y);
-#pragma unroll
-    for (int f = 0; f < G; ++f) dt[f] = fma(tl, y[f] - xc[f], dt[f]);
+    for (int q = 0; q < 13; ++q) o[q] = __mul24(p[q], F);
+    grad2_pts<F, G, T>(x, o, T(i), T(j), T(k), T(l), dr, ds, dt);
 }
 
 // The L0 stencil of l0_prep for node records of F values: record offsets (index times F)
@@ -273,25 +306,48 @@ template <typename T> struct L0Rec {
     int o0, o1, o2, o3, o4, o5, o6;
     T w0, w1, w2, w3, w4, w5, w6;
 };
-template <int N, int F, typename T> __device__ __forceinline__ L0Rec<T> l0_prep_rec(int a, int b) {
+// The 7 points of the L0 stencil at face node (a,b) of degree N (face-record indices:
+// the node, then (a+1,b-1), (a+1,b), (a-1,b+1), (a,b+1), (a-1,b), (a,b-1) in linear order,
+// each replaced by the node where its weight is zero) and their integer weights, without
+// the factor 1/2 of L0.  Shared by the kernels' run-time offsets and the per-degree tables.
+__host__ __device__ constexpr void l0_points(int N, int a, int b, int (&p)[7], int (&w)[7]) {
     const int c = N - a - b;
-    const int cF = __mul24(lin2(N, a, b), F), rlF = __mul24(N - b + 1, F);
-    L0Rec<T> p;
-    p.o0 = cF;
-    p.o1 = b > 0 ? cF - rlF : cF;
-    p.o2 = c > 0 ? cF + F : cF;
-    p.o3 = a > 0 ? cF + rlF - F : cF;
-    p.o4 = c > 0 ? cF + rlF : cF;
-    p.o5 = a > 0 ? cF - F : cF;
-    p.o6 = b > 0 ? cF - rlF - F : cF;
-    p.w0 = T((a + 1) * (a + 1) + (b + 1) * (b + 1) + (c + 1) * (c + 1));
-    p.w1 = T((a + 1) * b);
-    p.w2 = T((a + 1) * c);
-    p.w3 = T((b + 1) * a);
-    p.w4 = T((b + 1) * c);
-    p.w5 = T((c + 1) * a);
-    p.w6 = T((c + 1) * b);
-    return p;
+    const int ci = lin2(N, a, b), rl = N - b + 1;
+    p[0] = ci;
+    p[1] = b > 0 ? ci - rl : ci;
+    p[2] = c > 0 ? ci + 1 : ci;
+    p[3] = a > 0 ? ci + rl - 1 : ci;
+    p[4] = c > 0 ? ci + rl : ci;
+    p[5] = a > 0 ? ci - 1 : ci;
+    p[6] = b > 0 ? ci - rl - 1 : ci;
+    w[0] = (a + 1) * (a + 1) + (b + 1) * (b + 1) + (c + 1) * (c + 1);
+    w[1] = (a + 1) * b;
+    w[2] = (a + 1) * c;
+    w[3] = (b + 1) * a;
+    w[4] = (b + 1) * c;
+    w[5] = (c + 1) * a;
+    w[6] = (c + 1) * b;
+}
+
+template <int N, int F, typename T> __device__ __forceinline__ L0Rec<T> l0_prep_rec(int a, int b) {
+    int p[7], w[7];
+    l0_points(N, a, b, p, w);
+    L0Rec<T> r;
+    r.o0 = __mul24(p[0], F);
+    r.o1 = __mul24(p[1], F);
+    r.o2 = __mul24(p[2], F);
+    r.o3 = __mul24(p[3], F);
+    r.o4 = __mul24(p[4], F);
+    r.o5 = __mul24(p[5], F);
+    r.o6 = __mul24(p[6], F);
+    r.w0 = T(w[0]);
+    r.w1 = T(w[1]);
+    r.w2 = T(w[2]);
+    r.w3 = T(w[3]);
+    r.w4 = T(w[4]);
+    r.w5 = T(w[5]);
+    r.w6 = T(w[6]);
+    return r;
 }
 
 // Weight of slab s of the unnormalised lift chain (stage and lift kernels): slab 0 is 2 L0 x
