@@ -1,12 +1,12 @@
 // bb_batch.hip -- many independent meshes whose elements carry their own degree, stepped
 // together: the elements are stored grouped by degree (group N: all elements of degree N,
-// state [K_N][F][Np(N)] FP64, groups concatenated), and every LSRK45 stage is one launch of
-// the fused stage kernel per degree present, all reading the previous stage's state of
-// every group (ping-pong buffers) and writing the next.  A face between elements of
-// different degree reads the neighbour's whole face trace and brings it to the element's
-// degree with a host-built transfer matrix (degree elevation from below, L2 projection
-// from above) before the usual upwind flux.  Steps are captured into hipGraphs; the
-// tiling of each degree group is autotuned separately.
+// state [K_N][F][Np(N)] FP64, groups concatenated).  Every LSRK45 stage is one launch of the
+// transfer pass (k_xtrace: the face trace of every neighbour of another degree, brought to
+// the element's degree with a host-built transfer matrix -- degree elevation from below, L2
+// projection from above -- and written in the element's face node order), then one launch
+// of the fused stage kernel per degree present, all reading the previous stage's state of
+// every group (ping-pong buffers) and writing the next.  Steps are captured into hipGraphs;
+// the tiling of each degree group is autotuned separately.
 #include <math.h>
 #include <string.h>
 
@@ -32,19 +32,29 @@ struct bb_batch {
     int64_t K = 0, words = 0;
     double sign[6] = {1, 1, 1, 1, 1, 1};
     std::vector<BatchGroup> groups;
-    double *geo = nullptr, *invJ = nullptr, *nJ = nullptr, *dte = nullptr, *tr = nullptr, *evd = nullptr;
+    double *geo = nullptr, *invJ = nullptr, *nJ = nullptr, *dte = nullptr;
     float *geo32 = nullptr;  // geo / 2 in FP32 (mixed precision); geo holds geo / 2 in FP64
-    float *trf = nullptr, *evf = nullptr;
-    int32_t *erow = nullptr;
-    uint8_t *ecol = nullptr;
     int2 *conn = nullptr;
     int64_t *nboff = nullptr;
+    // the traces between elements of different degree (see BatchHost): the transfer pass's
+    // chunks and pair matrices (A-operand layout), the trace buffer the stage kernels read,
+    // the send buffers (traces of q[0] / q[1], written by the stage kernels that produce that
+    // state), and the per-face arrays of the pass that fills a send buffer from a state
+    int64_t nchunk = 0, nx = 0;
+    XChunk *xchunk = nullptr;
+    int64_t *xoff = nullptr, *soff = nullptr, *xnbq = nullptr, *xsrc = nullptr;
+    int32_t *xcode = nullptr;
+    double *xa64 = nullptr;
+    float *xa32 = nullptr;
+    double *xbuf = nullptr;                   // xwords values: FP64, or FP32 in the first half
+    double *send[2] = {nullptr, nullptr};     // swords values each, likewise
+    int send_prec = -1;                       // the precision send[cur] holds (-1: none)
     double *q[2] = {nullptr, nullptr};
     int cur = 0;
     double *res64 = nullptr;
     float *res32 = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    int g_flip = 0;
+    int g_flip = 0, g_prec = 0;
     bool g_ok = false;
 };
 
@@ -74,14 +84,197 @@ static StageArgs batch_args(bb_batch *b, const BatchGroup &g, int prec, int cur)
     for (int f = 0; f < 6; ++f) a.sign[f] = b->sign[f];
     a.q_nb = b->q[cur];
     a.nboff = b->nboff + g.start * 4;
-    a.tr = b->tr;
     a.dte = b->dte + g.start;
-    a.trf = b->trf;
-    a.erow = b->erow;
-    a.ecol = b->ecol;
-    a.evd = b->evd;
-    a.evf = b->evf;
+    a.xbuf = b->xbuf;
+    a.xoff = b->xoff + g.start * 4;
+    a.xsend = b->send[cur ^ 1];
+    a.soff = b->soff + g.start * 4;
     return a;
+}
+
+// r -> m: the node m of a degree-d face triangle whose node in the neighbour's orientation
+// (face permutation p: (a, b, c) of m -> (a2, b2) = its entries perm_entry(p, 0) and
+// perm_entry(p, 1)) is r, d = 1..10
+struct PermInv {
+    uint8_t v[11][6][kTraceMax];
+    constexpr PermInv() : v{} {
+        for (int d = 1; d <= 10; ++d)
+            for (int p = 0; p < 6; ++p)
+                for (int b = 0; b <= d; ++b)
+                    for (int a = 0; a <= d - b; ++a) {
+                        const int x[3] = {a, b, d - a - b};
+                        v[d][p][lin2(d, x[perm_entry(p, 0)], x[perm_entry(p, 1)])] = (uint8_t)lin2(d, a, b);
+                    }
+    }
+};
+__device__ constexpr PermInv kPermInv = PermInv();
+
+// The 16x16x4 matrix-core products: lane L supplies A[L % 16][L / 16] and B[L / 16][L % 16]
+// and holds four entries of column L % 16 of D: rows 4 (L / 16) + e in FP32, rows 4 e + L / 16
+// in FP64 (the FP64 product interleaves its output rows)
+template <typename T> struct XMfma;
+template <> struct XMfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int L, int e) { return 4 * (L >> 4) + e; }
+};
+template <> struct XMfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int L, int e) { return 4 * e + (L >> 4); }
+};
+
+// The transfer pass (one launch per stage, before the degree groups): for every cross face,
+// the neighbour's face trace (degree Nn, its face and node order: send buffer, written by
+// the neighbour's previous stage) brought to the face's degree N and written in the face's
+// own node order, xbuf[out + m F + f], in the kernels' type T.  A workgroup takes a chunk of
+// at most kXChunk faces of one degree pair, whose traces are one contiguous range:
+//   1. the pair's A tiles (W, Nfp(N) x Nfp(Nn)) and the chunk's traces (to tr[face][j][f],
+//      T) straight into LDS; the k-step padding j in [Nfp(Nn), 4 KS) zeroed;
+//   2. wave w multiplies W by the 16 columns (face 4w + n / 4, field n % 4) of tr on the
+//      matrix cores (16x16x4, FP32 or FP64 accumulation), RT row tiles over the k steps;
+//   3. row r of column (face, f) is written to node kPermInv[N][perm][r] of the face.
+// Bounds: the host builds every offset from the arrays it uploads (N, Nn in 1..10); a chunk
+// reads nc Nfp(Nn) F values from src0 and writes nc Nfp(N) F values from out0.
+constexpr int kXFs = 4 * 68 + 16;  // LDS stride of a face's trace: 4 x (66 nodes, padded to 68) + 16
+constexpr int xa_tiles_max() {
+    int m = 0;
+    for (int N = 1; N <= 10; ++N)
+        for (int Nn = 1; Nn <= 10; ++Nn) {
+            const int t = (nfp2(N) + 15) / 16 * ((nfp2(Nn) + 3) / 4);
+            if (Nn != N && t > m) m = t;
+        }
+    return m;
+}
+constexpr int kXATiles = xa_tiles_max();  // 70: N = 10 <- 9
+typedef __attribute__((address_space(3))) void lds_void;
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_xtrace(const XChunk *__restrict__ chunk, int64_t nchunk,
+                                                const void *__restrict__ send, const void *__restrict__ xa,
+                                                void *__restrict__ xbuf) {
+    using T = std::conditional_t<MIXED, float, double>;
+    using M = XMfma<T>;
+    constexpr int F = 4;
+    __shared__ __align__(16) T tr[kXChunk * kXFs];
+    __shared__ __align__(16) T as[kXATiles * 64];
+    const int tid = threadIdx.x, w = tid >> 6, L = tid & 63;
+    for (int64_t c = xcd_block(blockIdx.x, gridDim.x); c < nchunk; c += gridDim.x) {
+        const XChunk cd = chunk[c];
+        const int nc = cd.code & 255, N = (cd.code >> 8) & 15, Nn = (cd.code >> 12) & 15;
+        const int nf = nfp2(N), nfn = nfp2(Nn);
+        const int KS = (nfn + 3) / 4, nfnp = 4 * KS, RT = (nf + 15) / 16, NA = RT * KS * 64;
+        __syncthreads();  // the previous chunk's reads of tr and as are done
+        // 16-byte pieces straight into LDS, lane L of a wave to its destination + 16 L
+        {
+            const char *ag = reinterpret_cast<const char *>(xa) + (size_t)cd.aoff * sizeof(T);
+            const int P = NA * (int)sizeof(T) / 16;
+            for (int p0 = 64 * w; p0 < P; p0 += 256)
+                if (p0 + L < P)
+                    __builtin_amdgcn_global_load_lds(ag + (size_t)(p0 + L) * 16, (lds_void *)(as + p0 * 16 / sizeof(T)),
+                                                     16, 0, 0);
+            // face k's trace: nfn F values (a multiple of 16 bytes) from src0 + k nfn F
+            const char *sg = reinterpret_cast<const char *>(send) + (size_t)cd.src0 * sizeof(T);
+            const int PF = nfn * F * (int)sizeof(T) / 16;
+            for (int k = w; k < nc; k += 4) {
+                const char *sk = sg + (size_t)k * PF * 16;
+                for (int p0 = 0; p0 < PF; p0 += 64)
+                    if (p0 + L < PF)
+                        __builtin_amdgcn_global_load_lds(sk + (size_t)(p0 + L) * 16,
+                                                         (lds_void *)(tr + k * kXFs + p0 * 16 / sizeof(T)), 16, 0, 0);
+            }
+        }
+        // the k-step padding of every face (at most 3 nodes of F values)
+        for (int t = tid; t < nc * 12; t += 256) {
+            const int k = t / 12, q = t - k * 12;
+            if (4 * nfn + q < 4 * nfnp) tr[k * kXFs + 4 * nfn + q] = T(0);
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's pieces are in LDS
+        __syncthreads();
+        if (4 * w < nc) {
+            const T *ap = as + L;
+            const T *bp = tr + (4 * w + ((L & 15) >> 2)) * kXFs + 4 * (L >> 4) + (L & 3);
+            typename M::v4 acc[5];
+#pragma unroll
+            for (int rt = 0; rt < 5; ++rt) acc[rt] = typename M::v4{T(0), T(0), T(0), T(0)};
+            for (int ks = 0; ks < KS; ++ks) {
+                const T b = bp[16 * ks];
+#pragma unroll
+                for (int rt = 0; rt < 5; ++rt)
+                    if (rt < RT) acc[rt] = M::op(ap[(ks * RT + rt) * 64], b, acc[rt]);
+            }
+            const int k = 4 * w + ((L & 15) >> 2);
+            if (k < nc) {
+                const uint8_t *pi = kPermInv.v[N][(cd.perms >> (3 * k)) & 7];
+                T *out = reinterpret_cast<T *>(xbuf) + cd.out0 + (int64_t)k * nf * F + (L & 3);
+#pragma unroll
+                for (int rt = 0; rt < 5; ++rt)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int r = 16 * rt + M::row(L, e);
+                        if (rt < RT && r < nf) out[pi[r] * F] = acc[rt][e];
+                    }
+            }
+        }
+    }
+}
+
+// The send buffer from a state (after the state was set, or in another precision): one
+// thread per (cross face i, node j of the neighbour's face): node j of face fn of the
+// neighbour (state at q + xnbq[i], degree Nn, code fn | Nn << 2) -> send[xsrc[i] + j F + f].
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_xsend_gather(const double *__restrict__ q, int64_t nx,
+                                                      const int64_t *__restrict__ xnbq,
+                                                      const int64_t *__restrict__ xsrc,
+                                                      const int32_t *__restrict__ xcode, void *__restrict__ send) {
+    using T = std::conditional_t<MIXED, float, double>;
+    constexpr int F = 4;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nx * kTraceMax; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / kTraceMax;
+        const int j = (int)(t - i * kTraceMax);
+        const int code = xcode[i], fn = code & 3, Nn = (code >> 2) & 15;
+        if (j >= nfp2(Nn)) continue;
+        int a, b;
+        decode2(Nn, j, a, b);
+        const double *qj = q + xnbq[i] + face_vidx(Nn, fn, a, b);
+        T *d = reinterpret_cast<T *>(send) + xsrc[i] + j * F;
+#pragma unroll
+        for (int f = 0; f < F; ++f) d[f] = T(qj[f * np3(Nn)]);
+    }
+}
+
+// the traces of q[cur] in the send buffer, in precision prec (a no-op where they are)
+static int batch_send_ready(bb_batch *b, int prec) {
+    if (b->send_prec == prec || !b->nx) return BB_OK;
+    const int64_t items = b->nx * kTraceMax;
+    const int64_t blocks = (items + 255) / 256;
+    const unsigned grid = (unsigned)(blocks < (1 << 20) ? blocks : (1 << 20));
+    if (prec == BB_PREC_MIXED)
+        k_xsend_gather<true><<<grid, 256, 0, b->ctx->stream>>>(b->q[b->cur], b->nx, b->xnbq, b->xsrc, b->xcode,
+                                                               b->send[b->cur]);
+    else
+        k_xsend_gather<false><<<grid, 256, 0, b->ctx->stream>>>(b->q[b->cur], b->nx, b->xnbq, b->xsrc, b->xcode,
+                                                                b->send[b->cur]);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bb_fail(b->ctx, BB_ERR_HIP, "trace gather launch failed: %s", hipGetErrorString(e));
+    b->send_prec = prec;
+    return BB_OK;
+}
+
+// the transfer pass on the traces of q[cur]
+static int batch_xtrace(bb_batch *b, int prec, int cur) {
+    if (!b->nchunk) return BB_OK;
+    const unsigned grid = (unsigned)(b->nchunk < (1 << 22) ? b->nchunk : (1 << 22));
+    if (prec == BB_PREC_MIXED)
+        k_xtrace<true><<<grid, 256, 0, b->ctx->stream>>>(b->xchunk, b->nchunk, b->send[cur], b->xa32, b->xbuf);
+    else
+        k_xtrace<false><<<grid, 256, 0, b->ctx->stream>>>(b->xchunk, b->nchunk, b->send[cur], b->xa64, b->xbuf);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bb_fail(b->ctx, BB_ERR_HIP, "transfer pass launch failed: %s", hipGetErrorString(e));
+    return BB_OK;
 }
 
 static int batch_launch(bb_batch *b, const BatchGroup &g, int prec, int tiling, const StageArgs &a) {
@@ -126,11 +319,13 @@ static int batch_prepare(bb_batch *b, int prec) {
     return BB_OK;
 }
 
-// enqueue nsteps steps (capturable): per stage, one launch per degree group
+// enqueue nsteps steps (capturable): per stage, the transfer pass and one launch per degree
+// group; the send buffer must hold the traces of q[*cur] (batch_send_ready)
 static int batch_enqueue(bb_batch *b, int prec, double dt_scale, int nsteps, int *cur) {
     int rc;
     for (int st = 0; st < nsteps; ++st)
         for (int stage = 0; stage < 5; ++stage) {
+            if ((rc = batch_xtrace(b, prec, *cur))) return rc;
             for (auto &g : b->groups) {
                 StageArgs a = batch_args(b, g, prec, *cur);
                 a.ca = kLsrkA[stage];
@@ -197,18 +392,23 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
     };
     hipError_t err;
     const int64_t words = h.words;
-    const int64_t tl = transfer_len + kTransferPad;
+    const size_t nx = h.xnbq.size(), nch = h.xchunk.size();
     if ((err = hipMalloc(&b->geo, K * 9 * sizeof(double))) || (err = hipMalloc(&b->invJ, K * sizeof(double))) ||
         (err = hipMalloc(&b->geo32, K * 9 * sizeof(float))) ||
         (err = hipMalloc(&b->nJ, K * 20 * sizeof(double))) || (err = hipMalloc(&b->dte, K * sizeof(double))) ||
         (err = hipMalloc(&b->conn, K * 4 * sizeof(int2))) || (err = hipMalloc(&b->nboff, K * 4 * sizeof(int64_t))) ||
-        (err = hipMalloc(&b->tr, tl * sizeof(double))) ||
         (err = hipMalloc(&b->q[0], words * sizeof(double))) ||
-        (err = hipMalloc(&b->trf, tl * sizeof(float))) ||
-        (err = hipMalloc(&b->erow, (h.erow.size() + 1) * sizeof(int32_t))) ||
-        (err = hipMalloc(&b->ecol, h.ecol.size() + 1)) ||
-        (err = hipMalloc(&b->evd, (h.evd.size() + 1) * sizeof(double))) ||
-        (err = hipMalloc(&b->evf, (h.evf.size() + 1) * sizeof(float))) ||
+        (err = hipMalloc(&b->xoff, K * 4 * sizeof(int64_t))) ||
+        (err = hipMalloc(&b->soff, K * 4 * sizeof(int64_t))) ||
+        (err = hipMalloc(&b->xchunk, (nch + 1) * sizeof(XChunk))) ||
+        (err = hipMalloc(&b->xnbq, (nx + 1) * sizeof(int64_t))) ||
+        (err = hipMalloc(&b->xsrc, (nx + 1) * sizeof(int64_t))) ||
+        (err = hipMalloc(&b->xcode, (nx + 1) * sizeof(int32_t))) ||
+        (err = hipMalloc(&b->send[0], (h.swords + 1) * sizeof(double))) ||
+        (err = hipMalloc(&b->send[1], (h.swords + 1) * sizeof(double))) ||
+        (err = hipMalloc(&b->xa64, (h.xa64.size() + 1) * sizeof(double))) ||
+        (err = hipMalloc(&b->xa32, (h.xa32.size() + 1) * sizeof(float))) ||
+        (err = hipMalloc(&b->xbuf, (h.xwords + 1) * sizeof(double))) ||
         (err = hipMalloc(&b->q[1], words * sizeof(double))))
         return fail(err);
     if ((err = hipMemcpy(b->geo, h.geoh.data(), K * 9 * sizeof(double), hipMemcpyHostToDevice)) ||
@@ -218,20 +418,17 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
         (err = hipMemcpy(b->dte, dt_host, K * sizeof(double), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->conn, h.conn.data(), K * 4 * sizeof(int2), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->nboff, h.nboff.data(), K * 4 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->tr, transfer_host, transfer_len * sizeof(double), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->trf, h.trf.data(), h.trf.size() * sizeof(float), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->erow, h.erow.data(), h.erow.size() * sizeof(int32_t), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->ecol, h.ecol.data(), h.ecol.size(), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->evd, h.evd.data(), h.evd.size() * sizeof(double), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->evf, h.evf.data(), h.evf.size() * sizeof(float), hipMemcpyHostToDevice)))
+        (err = hipMemcpy(b->xoff, h.xoff.data(), K * 4 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->soff, h.soff.data(), K * 4 * sizeof(int64_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->xchunk, h.xchunk.data(), nch * sizeof(XChunk), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->xnbq, h.xnbq.data(), nx * sizeof(int64_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->xsrc, h.xsrc.data(), nx * sizeof(int64_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->xcode, h.xcode.data(), nx * sizeof(int32_t), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->xa64, h.xa64.data(), h.xa64.size() * sizeof(double), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->xa32, h.xa32.data(), h.xa32.size() * sizeof(float), hipMemcpyHostToDevice)))
         return fail(err);
-    {
-        const int rc = bb_fill_zero(ctx, b->tr + transfer_len, kTransferPad * sizeof(double));
-        if (rc) {
-            bb_batch_destroy(b);
-            return rc;
-        }
-    }
+    b->nchunk = (int64_t)nch;
+    b->nx = (int64_t)nx;
     for (size_t i = 0; i < b->groups.size(); ++i) {
         BatchGroup &g = b->groups[i];
         const std::vector<uint16_t> &tab = tabs[i];
@@ -255,8 +452,9 @@ int bb_batch_destroy(bb_batch *b) {
     if (!b) return BB_ERR_NULL;
     hipStreamSynchronize(b->ctx->stream);
     batch_drop_graphs(b);
-    void *ptrs[] = {b->geo, b->geo32, b->invJ, b->nJ, b->dte, b->tr, b->conn, b->nboff, b->q[0], b->q[1], b->res64, b->res32,
-                    b->trf, b->erow, b->ecol, b->evd, b->evf};
+    void *ptrs[] = {b->geo,   b->geo32, b->invJ, b->nJ,   b->dte,  b->conn, b->nboff, b->q[0],   b->q[1],
+                    b->res64, b->res32, b->xoff, b->soff, b->xchunk, b->xnbq, b->xsrc, b->xcode, b->xa64,
+                    b->xa32,  b->xbuf,  b->send[0], b->send[1]};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &g : b->groups) {
@@ -280,6 +478,7 @@ int bb_batch_set_state(bb_batch *b, const double *q_host) {
     BB_HIP(b->ctx, hipMemcpyAsync(b->q[b->cur], q_host, b->words * sizeof(double), hipMemcpyHostToDevice,
                                   b->ctx->stream));
     BB_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+    b->send_prec = -1;  // the traces of the new state are gathered before the next use
     return BB_OK;
 }
 
@@ -304,6 +503,7 @@ static int batch_rhs(bb_batch *b, int prec, double *du_host) {
     int rc = batch_prepare(b, prec);
     if (rc) return rc;
     bb_ctx *ctx = b->ctx;
+    if ((rc = batch_send_ready(b, prec)) || (rc = batch_xtrace(b, prec, b->cur))) return rc;
     for (auto &g : b->groups) {
         StageArgs a = batch_args(b, g, prec, b->cur);
         a.q_out = nullptr;
@@ -328,7 +528,7 @@ int bb_batch_step(bb_batch *b, int prec, double dt_scale, int nsteps) {
     if (!b) return BB_ERR_NULL;
     if (nsteps < 0) return bb_fail(b->ctx, BB_ERR_ARG, "nsteps %d", nsteps);
     int rc = batch_prepare(b, prec);
-    if (rc) return rc;
+    if (rc || (rc = batch_send_ready(b, prec))) return rc;
     return batch_enqueue(b, prec, dt_scale, nsteps, &b->cur);
 }
 
@@ -359,6 +559,7 @@ int bb_batch_graph_capture(bb_batch *b, int prec, double dt_scale, int nsteps) {
         if (e != hipSuccess) return bb_fail(ctx, BB_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
     }
     b->g_flip = flip;
+    b->g_prec = prec;
     b->g_ok = true;
     return BB_OK;
 }
@@ -366,6 +567,8 @@ int bb_batch_graph_capture(bb_batch *b, int prec, double dt_scale, int nsteps) {
 int bb_batch_graph_launch(bb_batch *b) {
     if (!b) return BB_ERR_NULL;
     if (!b->g_ok || !b->gexec[b->cur]) return bb_fail(b->ctx, BB_ERR_STATE, "no graph captured for this buffer");
+    const int rc = batch_send_ready(b, b->g_prec);  // outside the graph: only after a new state
+    if (rc) return rc;
     BB_HIP(b->ctx, hipGraphLaunch(b->gexec[b->cur], b->ctx->stream));
     b->cur ^= b->g_flip;
     return BB_OK;
@@ -389,6 +592,8 @@ int bb_batch_autotune(bb_batch *b, int prec, int *tilings) {
     int rc = batch_prepare(b, prec);
     if (rc) return rc;
     bb_ctx *ctx = b->ctx;
+    // the traces the group kernels read
+    if ((rc = batch_send_ready(b, prec)) || (rc = batch_xtrace(b, prec, b->cur))) return rc;
     for (auto &g : b->groups) {
         StageArgs a = batch_args(b, g, prec, b->cur);  // writes the other buffer only
         a.ca = kLsrkA[1];

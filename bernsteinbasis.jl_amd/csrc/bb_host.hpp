@@ -7,6 +7,8 @@
 #pragma once
 #include <math.h>
 
+#include <algorithm>
+#include <tuple>
 #include <vector>
 
 #include "bb_stage.hpp"
@@ -101,6 +103,17 @@ inline int solver_host_setup(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t
     return BB_OK;
 }
 
+constexpr int kXChunk = 16;  // cross faces of one degree pair per transfer-pass workgroup
+constexpr int kXBlock = 64;  // neighbour elements whose cross faces are ordered together
+
+// a chunk of the transfer pass: its faces' traces at send[src0 ..] (Nfp(Nn) F values per
+// face), results to xbuf[out0 ..] (Nfp(N) F values per face), face k's perm in bits 3k..3k+2
+// of perms, nc faces | N << 8 | Nn << 12, the pair's A tiles at xa[aoff]
+struct alignas(16) XChunk {
+    int64_t src0, out0, perms;
+    int32_t code, aoff;
+};
+
 struct BatchHostGroup {
     int N = 0;
     int64_t start = 0, count = 0, qoff = 0;  // first element, elements, state offset (doubles)
@@ -112,20 +125,36 @@ struct BatchHost {
     std::vector<BatchHostGroup> groups;     // elements grouped by degree
     std::vector<int2> conn;                 // [K][4]
     std::vector<int64_t> nboff;             // [K][4] the neighbour's state offset
+    // The traces between elements of different degree.  A face whose neighbour has another
+    // degree (a "cross face") receives the neighbour's trace brought to its own degree: W
+    // (Nfp(N) x Nfp(Nn), the transfer matrix of the degree pair N <- Nn) times the trace.
+    //  * The cross faces are ordered by (Nn, neighbour element / kXBlock, N, neighbour state
+    //    offset) -- neighbours close together, for every degree pair -- and cut into chunks of
+    //    at most kXChunk consecutive faces of one pair (XChunk).
+    //  * The sending side: the neighbour's face trace, Nfp(Nn) records of F values in its own
+    //    face node order, is written by the neighbour's stage kernel (with its new state) to
+    //    the send buffer at soff[4 n + fn] (values; -1: that face sends nothing), the slots
+    //    of the cross faces one after another in the order above -- so the traces of a chunk
+    //    form one contiguous range.
+    //  * The transfer pass multiplies each chunk's traces by W on the matrix cores: W of each
+    //    pair present in the A-operand layout of the 16x16x4 product, xa[off + (ks RT + rt) 64
+    //    + L] = W[16 rt + L % 16][4 ks + L / 16] (zero outside W; RT = ceil(Nfp(N) / 16) row
+    //    tiles, ks over ceil(Nfp(Nn) / 4) k steps), in FP64 (xa64) and FP32 (xa32); and writes
+    //    the result in the receiving face's node order (its perm) to the trace buffer, the
+    //    cross faces' Nfp(N) records one after another in the order above: xoff[4 e + f] for
+    //    the receiving stage kernel (0 for the other faces).
+    //  * Per cross face, for the pass that fills the send buffer from a state (xsend_gather):
+    //    the neighbour's state offset xnbq, its send slot xsrc, fn | Nn << 2 (xcode).
+    std::vector<XChunk> xchunk;
+    std::vector<int64_t> xnbq, xsrc, xoff, soff;
+    std::vector<int32_t> xcode;
+    int64_t xwords = 0, swords = 0;         // trace and send buffer lengths (values)
+    std::vector<double> xa64;
+    std::vector<float> xa32;
     std::vector<double> nj5;                // [K][4][5]
     std::vector<double> invJ, geoh;         // [K], [K][9] geo / 2
-    std::vector<float> geoh32, trf;         // [K][9]; the transfer tables in FP32
-    // degree elevations (neighbour degree below the element's) as sparse rows:
-    // eoff[11 N + Nn] -> first row of erow; row r's entries are ecol/evd[erow[r] .. erow[r+1])
-    std::vector<int32_t> erow;
-    std::vector<int64_t> eoff;
-    std::vector<uint8_t> ecol;
-    std::vector<double> evd;
-    std::vector<float> evf;
+    std::vector<float> geoh32;              // [K][9]
 };
-
-// zero entries past the end of the device transfer tables (projection rows are read ahead)
-constexpr int kTransferPad = 8;
 
 // is the transfer matrix from degree Nn to N present in a table of transfer_len doubles?
 inline bool transfer_present(const int64_t *toff, int64_t transfer_len, int N, int Nn) {
@@ -189,42 +218,75 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
         g.count = e - g.start;
         h.groups.push_back(g);
     }
-    h.eoff.assign(121, -1);
-    for (int N = 1; N <= kStageMaxN; ++N)
-        for (int Nn = 1; Nn < N; ++Nn) {
-            if (!transfer_present(toff, transfer_len, N, Nn)) continue;
-            const int64_t o = toff[11 * N + Nn];
-            h.eoff[11 * N + Nn] = (int64_t)h.erow.size();
-            for (int r = 0; r < nfp2(N); ++r) {
-                h.erow.push_back((int32_t)h.evd.size());
-                for (int c = 0; c < nfp2(Nn); ++c) {
-                    const double v = transfer[o + (int64_t)r * nfp2(Nn) + c];
-                    if (v != 0.0) {
-                        h.ecol.push_back((uint8_t)c);
-                        h.evd.push_back(v);
-                    }
-                }
-            }
-            h.erow.push_back((int32_t)h.evd.size());
-        }
-    h.evf.assign(h.evd.begin(), h.evd.end());
-    // conn.x of an interior face whose neighbour has another degree holds where that face's
-    // transfer rows start (the kernels reach each element's neighbour through nboff): the
-    // first entry of the projection matrix in the transfer table for a higher neighbour
-    // degree, the first row of the sparse elevation rows for a lower one; 0 otherwise
+    // every degree pair present, in the A-operand layout
+    std::vector<int64_t> xaoff(121, -1);
     for (int64_t t = 0; t < K * 4; ++t) {
         const int N = deg[t / 4], Nn = deg[nbr[t]];
-        int64_t base = 0;
-        if (!bc[t] && Nn > N) base = toff[11 * N + Nn];
-        if (!bc[t] && Nn < N) base = h.eoff[11 * N + Nn];
-        if (base < 0 || base > INT32_MAX)
-            return bb_fail(ctx, BB_ERR_ARG, "transfer table offset %lld out of range", (long long)base);
-        h.conn[t].x = (int)base;
+        if (bc[t] || Nn == N || xaoff[11 * N + Nn] >= 0) continue;
+        const int nf = nfp2(N), nfn = nfp2(Nn), RT = (nf + 15) / 16, KS = (nfn + 3) / 4;
+        const int64_t o = toff[11 * N + Nn], a0 = (int64_t)h.xa64.size();
+        xaoff[11 * N + Nn] = a0;
+        h.xa64.resize(a0 + (int64_t)KS * RT * 64, 0.0);
+        for (int ks = 0; ks < KS; ++ks)
+            for (int rt = 0; rt < RT; ++rt)
+                for (int L = 0; L < 64; ++L) {
+                    const int r = 16 * rt + L % 16, c = 4 * ks + L / 16;
+                    if (r < nf && c < nfn) h.xa64[a0 + ((int64_t)ks * RT + rt) * 64 + L] = transfer[o + (int64_t)r * nfn + c];
+                }
     }
-    // the table in FP32, with kTransferPad zeros past its end (the kernels' projection rows
-    // are read four entries ahead)
-    h.trf.assign(transfer, transfer + transfer_len);
-    h.trf.resize(transfer_len + kTransferPad, 0.f);
+    if ((int64_t)h.xa64.size() > INT32_MAX) return bb_fail(ctx, BB_ERR_ARG, "transfer tables too large");
+    h.xa32.assign(h.xa64.begin(), h.xa64.end());
+    // the cross faces, in chunks of one degree pair
+    {
+        std::vector<int64_t> cross;
+        for (int64_t t = 0; t < K * 4; ++t)
+            if (!bc[t] && deg[nbr[t]] != deg[t / 4]) cross.push_back(t);
+        auto key = [&](int64_t t) {
+            return std::make_tuple(deg[nbr[t]], nbr[t] / kXBlock, deg[t / 4], h.nboff[t], t);
+        };
+        std::sort(cross.begin(), cross.end(), [&](int64_t x, int64_t y) { return key(x) < key(y); });
+        h.xoff.assign(K * 4, 0);
+        h.soff.assign(K * 4, -1);
+        h.xnbq.resize(cross.size());
+        h.xsrc.resize(cross.size());
+        h.xcode.resize(cross.size());
+        int64_t w = 0, sw = 0;
+        for (size_t i = 0; i < cross.size(); ++i) {
+            const int64_t t = cross[i];
+            const int N = deg[t / 4], Nn = deg[nbr[t]];
+            const int64_t snd = (int64_t)nbr[t] * 4 + nbr_face[t];  // the sending face
+            // each sending face sends to one face only: the connectivity must be mutual
+            if (nbr[snd] != t / 4 || nbr_face[snd] != t % 4 || bc[snd])
+                return bb_fail(ctx, BB_ERR_ARG, "face %d of element %lld and its neighbour do not match",
+                               (int)(t % 4), (long long)(t / 4));
+            h.xnbq[i] = h.nboff[t];
+            h.xsrc[i] = sw;
+            h.soff[snd] = sw;
+            h.xoff[t] = w;
+            h.xcode[i] = nbr_face[t] | (Nn << 2);
+            // a chunk holds consecutive faces of one (Nn, neighbour block, N)
+            const int64_t p = i ? cross[i - 1] : 0;
+            const bool cont = i > 0 && (h.xchunk.back().code & 255) < kXChunk && deg[nbr[p]] == Nn &&
+                              nbr[p] / kXBlock == nbr[t] / kXBlock && deg[p / 4] == N;
+            if (cont) {
+                XChunk &c = h.xchunk.back();
+                c.perms |= (int64_t)perm[t] << (3 * (c.code & 255));
+                ++c.code;
+            } else {
+                XChunk c;
+                c.src0 = sw;
+                c.out0 = w;
+                c.perms = perm[t];
+                c.code = 1 | (N << 8) | (Nn << 12);
+                c.aoff = (int32_t)xaoff[11 * N + Nn];
+                h.xchunk.push_back(c);
+            }
+            w += (int64_t)h.F * nfp2(N);
+            sw += (int64_t)h.F * nfp2(Nn);
+        }
+        h.xwords = w;
+        h.swords = sw;
+    }
     h.invJ.resize(K);
     h.geoh.resize(K * 9);
     h.geoh32.resize(K * 9);

@@ -46,20 +46,20 @@ struct StageArgs {
     double sign[6];
     // elements of mixed degree (k_stage with MD = true): the neighbour across face f of
     // element e has its state at q_nb + nboff[4 e + f] and its degree Nn in bits 8..11 of
-    // conn.y; element e steps by dt * dte[e].  A trace of another degree is brought to this
-    // degree by a transfer matrix: for Nn > N the dense projection at tr + conn.x
-    // (nfp(N) x nfp(Nn), row-major; trf in FP32), for Nn < N the sparse elevation whose
-    // row m holds entries erow[conn.x + m] .. erow[conn.x + m + 1] - 1 of ecol (neighbour
-    // face node) and evd / evf (weight)
+    // conn.y; element e steps by dt * dte[e].  Where Nn differs from the element's degree,
+    // the neighbour's trace brought to this degree (by the transfer pass that runs before
+    // the stage, k_xtrace) is at xbuf + xoff[4 e + f]: Nfp records of F values (the
+    // kernel's type), in this face's node order.  A face f whose neighbour has another
+    // degree also sends: with the new state, the stage writes the face's own trace (Nfp
+    // records of F values in the kernel's type, its own face node order) to xsend +
+    // soff[4 e + f] (soff < 0: no neighbour of another degree)
     const double *q_nb;
     const int64_t *nboff;
-    const double *tr;
     const double *dte;
-    const float *trf;
-    const int32_t *erow;
-    const uint8_t *ecol;
-    const double *evd;
-    const float *evf;
+    const void *xbuf;
+    const int64_t *xoff;
+    void *xsend;
+    const int64_t *soff;
 };
 
 // Five-stage fourth-order 2N-storage RK (Carpenter & Kennedy, NASA TM-109112, 1994)
@@ -136,7 +136,7 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     static constexpr int TABG = (2 * NP + 28 * NFP + 3) / 4 * 4;  // ijk, dec, own / neighbour face nodes
     static constexpr int TAB = TABG + 4 * NP;                      // then the gather offsets
     static constexpr int TAB16 = (TAB + 7) / 8;    // the same in 16-byte words
-    static constexpr int R3 = MD ? EB * 4 * RS * kTraceMax : 0;  // neighbour traces of other degrees
+
     // index tables in LDS, or read from global memory (WAVE_GT, WAVE1)
     static constexpr bool TABL = LM != LIFT_WAVE_GT && LM != LIFT_WAVE1;
     // prologue loads issued in the order they are consumed (FP64 single-degree kernels)
@@ -146,7 +146,7 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     // wave-owned chains only, and without the pinning below)
     static constexpr bool PRE_GEO = EB == 1 && (!MD || WAVEF);
     static constexpr bool PRE_NJ = WAVEF;
-    static constexpr size_t lds = (size_t)(R0 + R1 + R2 + R3) * sizeof(T) + (TABL ? (size_t)TAB16 * 16 : 0) + (N + 1) * 8;
+    static constexpr size_t lds = (size_t)(R0 + R1 + R2) * sizeof(T) + (TABL ? (size_t)TAB16 * 16 : 0) + (N + 1) * 8;
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
     static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024 && (LM != LIFT_GATHER_MFMA || MIXED);
@@ -263,7 +263,6 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     constexpr int FXS = Cfg::FXA ? NP : NFP;
     __shared__ __align__(16) T s0[Cfg::WAVEF ? Cfg::R0 : 1];
     T *sst = Cfg::WAVEF ? s0 : s1;  // the element state in LDS: [element][node][field]
-    __shared__ __align__(16) T s3[MD ? Cfg::R3 : 1];
     __shared__ uint4 tab16[Cfg::TABL ? Cfg::TAB16 : 1];
 
     const uint16_t *tab = Cfg::TABL ? reinterpret_cast<const uint16_t *>(tab16) : A.tab;
@@ -494,86 +493,13 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                                           : A.q_in + (int64_t)cnv[r].x * F * NP + nvi;
 #pragma unroll
                     for (int f = 0; f < F; ++f) nb[r][f] = T(qn[f * NP]);
+                } else if (MD && ((cnv[r].y >> 5) & 7) == 0) {
+                    // a neighbour of another degree: its trace at this degree and node order,
+                    // from the transfer pass's buffer
+                    const T *xb = reinterpret_cast<const T *>(A.xbuf) + A.xoff[(e0 + e2) * 4 + phi] + m * F;
+#pragma unroll
+                    for (int f = 0; f < F; ++f) nb[r][f] = xb[f];
                 }
-            }
-        }
-    }
-    if constexpr (MD && EB <= 2) {
-        // whole face traces of neighbours of another degree -> s3[(element, face)][node][field].
-        // Face c = (element, face) of the block stages nfp(Nn) records when its neighbour is
-        // an interior element of degree Nn != N, none otherwise; the staged records of the
-        // block's faces form one list, dealt to the threads in order.  The connectivity of
-        // the block's faces is read once (uniform: scalar loads), and every neighbour load of
-        // a chunk of up to SCH items per thread is issued before the chunk's first LDS store,
-        // so that the loads' latencies overlap instead of following one another.
-        constexpr int NF = EB * 4;
-        int cy[NF], cend[NF];
-        int64_t nbo[NF];
-        int tot = 0;
-#pragma unroll
-        for (int c = 0; c < NF; ++c) {
-            const bool ok = (c >> 2) < ne;
-            const int64_t fc = (e0 + (ok ? (c >> 2) : 0)) * 4 + (c & 3);
-            cy[c] = A.conn[fc].y;
-            nbo[c] = A.nboff[fc];
-            const int Nn = (cy[c] >> 8) & 15;
-            tot += (ok && ((cy[c] >> 5) & 7) == 0 && Nn != N) ? nfp2(Nn) : 0;
-            cend[c] = tot;
-        }
-        constexpr int MAXI = (NF * kTraceMax + NT - 1) / NT;  // items per thread at most
-        constexpr int SCH = MAXI < 3 ? MAXI : 3;
-#pragma unroll 1
-        for (int i0 = 0; i0 < MAXI && i0 * NT < tot; i0 += SCH) {
-            double v[SCH][F];
-            int dst[SCH];
-#pragma unroll
-            for (int u = 0; u < SCH; ++u) {
-                const int t = tid + (i0 + u) * NT;
-                const bool ok = t < tot;
-                // the item's face: the first c with t < cend[c] (selects, no indexed registers)
-                int yc = cy[0], beg = 0;
-                int64_t oc = nbo[0];
-                int c = 0;
-#pragma unroll
-                for (int cc = 1; cc < NF; ++cc) {
-                    const bool past = t >= cend[cc - 1];
-                    yc = past ? cy[cc] : yc;
-                    oc = past ? nbo[cc] : oc;
-                    beg = past ? cend[cc - 1] : beg;
-                    c += past;
-                }
-                const int Nn = ok ? (yc >> 8) & 15 : 1, jn = ok ? t - beg : 0;
-                const int npn = ok ? np3(Nn) : 1;
-                // items past the list read the first values of the state (never stored)
-                const double *qn = A.q_nb + (ok ? oc + kFaceVidAll.v[Nn][yc & 3][jn] : 0);
-#pragma unroll
-                for (int f = 0; f < F; ++f) v[u][f] = qn[f * npn];
-                dst[u] = ok ? c * kTraceMax + jn : -1;
-            }
-#pragma unroll
-            for (int u = 0; u < SCH; ++u)
-                if (dst[u] >= 0) {
-                    T w[F];
-#pragma unroll
-                    for (int f = 0; f < F; ++f) w[f] = T(v[u][f]);
-                    recv_store<F, RS>(s3 + __mul24(dst[u], RS), w);
-                }
-        }
-    } else if constexpr (MD) {
-        // the same, item by item (blocks of four elements)
-        for (int t = tid; t < ne * 4 * kTraceMax; t += NT) {
-            const int e2 = t / (4 * kTraceMax), phi = (t / kTraceMax) & 3, jn = t % kTraceMax;
-            const int2 cn = A.conn[(e0 + e2) * 4 + phi];
-            const int Nn = (cn.y >> 8) & 15;
-            if (((cn.y >> 5) & 7) == 0 && Nn != N && jn < nfp2(Nn)) {
-                int a, b;
-                decode2(Nn, jn, a, b);
-                const int npn = np3(Nn);
-                const double *qn = A.q_nb + A.nboff[(e0 + e2) * 4 + phi] + face_vidx(Nn, cn.y & 3, a, b);
-                T v[F];
-#pragma unroll
-                for (int f = 0; f < F; ++f) v[f] = T(qn[f * npn]);
-                recv_store<F, RS>(s3 + __mul24((e2 * 4 + phi) * kTraceMax + jn, RS), v);
             }
         }
     }
@@ -608,68 +534,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             const int vi = Cfg::TABL ? fvid[phi * NFP + m] : fvp[r];
             T um[F], d[F], o[F];
             recv_load<F, RS>(sst + __mul24(e2 * NP + vi, RS), um);
-            if (MD && ((cnv[r].y >> 5) & 7) == 0 && ((cnv[r].y >> 8) & 15) != N) {
-                // the neighbour's trace of degree Nn brought to degree N (in the neighbour's
-                // face orientation), read at this node through the face permutation
-                const int Nn = (cnv[r].y >> 8) & 15, p = (cnv[r].y >> 2) & 7, nfn = nfp2(Nn);
-                const int ab = dec[np3(N - 1) + m];
-                const int a = ab & 255, b = ab >> 8, c = N - a - b;
-                const int q0 = perm_entry(p, 0), q1 = perm_entry(p, 1);
-                const int a2 = q0 == 0 ? a : (q0 == 1 ? b : c), b2 = q1 == 0 ? a : (q1 == 1 ? b : c);
-                const int mp = lin2(N, a2, b2);
-                const T *tn = s3 + __mul24(e2 * 4 + phi, kTraceMax * RS);
-                T up[F], tv[F];
-#pragma unroll
-                for (int f = 0; f < F; ++f) up[f] = T(0);
-                if (Nn < N) {
-                    // elevation: the few neighbour nodes below this one (sparse row).  The
-                    // unroll depths of these loops set the mixed-degree kernels' register
-                    // peak: in FP32, 2 and 4 (instead of 4 and 8) leave 71-81 VGPRs instead of
-                    // 82-103 and made the mixed batch 7% faster; FP64 keeps 4 and 8
-                    const int32_t *rp = A.erow + cnv[r].x + mp;  // conn.x: the pair's first row
-                    const int j1 = rp[1];
-#pragma unroll MIXED ? 2 : 4
-                    for (int jj = rp[0]; jj < j1; ++jj) {
-                        const T w = MIXED ? T(A.evf[jj]) : T(A.evd[jj]);
-                        recv_load<F, RS>(tn + __mul24(A.ecol[jj], RS), tv);
-#pragma unroll
-                        for (int f = 0; f < F; ++f) up[f] = fma(w, tv[f], up[f]);
-                    }
-                } else {
-                    // L2 projection from a higher degree: a dense row (conn.x: where the pair's
-                    // matrix starts), four entries at a time with the next four loaded ahead
-                    // (the tables end in kTransferPad zeros, so the read-ahead stays inside)
-                    const T *wr = (const T *)(MIXED ? (const void *)A.trf : (const void *)A.tr) + cnv[r].x + mp * nfn;
-                    T wn[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) wn[u] = wr[u];
-                    int jn = 0;
-#pragma unroll 1
-                    for (; jn + 4 <= nfn; jn += 4) {
-                        T wc[4];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            wc[u] = wn[u];
-                            wn[u] = wr[jn + 4 + u];
-                        }
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            recv_load<F, RS>(tn + __mul24(jn + u, RS), tv);
-#pragma unroll
-                            for (int f = 0; f < F; ++f) up[f] = fma(wc[u], tv[f], up[f]);
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 3; ++u)
-                        if (jn + u < nfn) {
-                            recv_load<F, RS>(tn + __mul24(jn + u, RS), tv);
-#pragma unroll
-                            for (int f = 0; f < F; ++f) up[f] = fma(wn[u], tv[f], up[f]);
-                        }
-                }
-#pragma unroll
-                for (int f = 0; f < F; ++f) d[f] = up[f] - um[f];
-            } else if (((cnv[r].y >> 5) & 7) == 0) {
+            if (((cnv[r].y >> 5) & 7) == 0) {
 #pragma unroll
                 for (int f = 0; f < F; ++f) d[f] = nb[r][f] - um[f];
             } else {
@@ -1001,6 +866,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
 
     // ---- epilogue: rhs = (lift + volume) / J, then the RK stage
     if (own) {
+        T qs[F];  // the new state in T, for the face traces sent to neighbours of another degree
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             const double rhs = (acc[f] + (double)vol[f]) * invJ;
@@ -1011,8 +877,24 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 // in the first stage rv is 0 (not loaded), so this is exactly dte * rhs
                 const double r = fma(A.ca, (double)rv[f], dte * rhs);
                 const TR rs = TR(r);
+                const double qn = qv[f] + A.cb * (double)rs;
                 __builtin_nontemporal_store(rs, &res[g]);
-                __builtin_nontemporal_store(qv[f] + A.cb * (double)rs, &A.q_out[g]);
+                __builtin_nontemporal_store(qn, &A.q_out[g]);
+                qs[f] = T(qn);
+            }
+        }
+        if constexpr (MD) {
+            if (!A.rhs_only) {
+                // this node is node ix[fc] of face fc where its slab index st[fc] is 0
+#pragma unroll
+                for (int fc = 0; fc < 4; ++fc) {
+                    const int64_t o = A.soff[eg * 4 + fc];
+                    if (st[fc] == 0 && o >= 0) {
+                        T *d = reinterpret_cast<T *>(A.xsend) + o + ix[fc] * F;
+#pragma unroll
+                        for (int f = 0; f < F; ++f) d[f] = qs[f];
+                    }
+                }
             }
         }
     }

@@ -149,12 +149,39 @@ static void check_batch_setup() {
                                toff.data(), tl, h) == BB_OK);
         CHECK(h.groups.size() == 2 && h.groups[1].qoff == 4 * np3(3) && h.words == 4 * (np3(3) + np3(4)));
         CHECK(h.nboff[0] == 4 * np3(3) && h.nboff[4] == 0 && (h.conn[0].y >> 8) == 4 && (h.conn[4].y >> 8) == 3);
-        CHECK(h.eoff[11 * 4 + 3] == 0 && h.erow.size() == (size_t)n4 + 1 && h.evd.size() == (size_t)n4);
-        CHECK(h.trf.size() == tr.size() + kTransferPad && h.evf.size() == h.evd.size());
-        CHECK(h.trf[tr.size()] == 0.f && h.trf[tr.size() + kTransferPad - 1] == 0.f);
-        // conn.x: where each face's transfer rows start (projection for element 0, whose
-        // neighbour has the higher degree; the elevation rows for element 1; 0 on walls)
-        CHECK(h.conn[0].x == n3 * n4 && h.conn[4].x == h.eoff[11 * 4 + 3] && h.conn[1].x == 0);
+        // the pair matrices in the A-operand layout of the 16x16x4 product: 3 <- 4 first (the
+        // first cross face in element order is (0, 0)), then 4 <- 3
+        const int RT3 = (n3 + 15) / 16, KS4 = (n4 + 3) / 4, RT4 = (n4 + 15) / 16, KS3 = (n3 + 3) / 4;
+        CHECK(h.xa64.size() == (size_t)64 * (RT3 * KS4 + RT4 * KS3) && h.xa32.size() == h.xa64.size());
+        for (int ks = 0; ks < KS4; ++ks)
+            for (int rt = 0; rt < RT3; ++rt)
+                for (int L = 0; L < 64; ++L) {
+                    const int r = 16 * rt + L % 16, c = 4 * ks + L / 16;
+                    const double want = (r < n3 && c < n4) ? tr[(size_t)n3 * n4 + (size_t)r * n4 + c] : 0.0;
+                    CHECK(h.xa64[(ks * RT3 + rt) * 64 + L] == want);
+                }
+        const size_t a43 = (size_t)64 * RT3 * KS4;
+        CHECK(h.xa64[a43 + 0] == tr[0] && h.xa64[a43 + 0] == 1.0);         // W[0][0]
+        CHECK(h.xa64[a43 + 5 + 16 * 1] == tr[(size_t)5 * n3 + 1]);            // W[5][1]: lane 21
+        CHECK(h.xa64[a43 + 64 + 3 + 16 * 2] == tr[(size_t)3 * n3 + 6]);       // W[3][6]: ks 1, lane 35
+        CHECK(h.xa64[a43 + 128 + 14 + 16 * 1] == tr[(size_t)14 * n3 + 9]);    // W[14][9]: ks 2
+        CHECK(h.xa64[a43 + 128 + 14 + 16 * 2] == 0.0);                        // column 10: padding
+        // conn.x is the neighbour element, as in the single-degree kernels
+        CHECK(h.conn[0].x == 1 && h.conn[4].x == 0);
+        // the cross faces, ordered by neighbour degree first: face (1, 0) receives from
+        // element 0 (degree 3) before face (0, 0) receives from element 1 (degree 4); one
+        // chunk per pair.  Send slots in that order: element 0 face 0 sends first (n3 nodes)
+        CHECK(h.xnbq.size() == 2 && h.xnbq[0] == 0 && h.xnbq[1] == 4 * np3(3));
+        CHECK(h.xcode[0] == (0 | (3 << 2)) && h.xcode[1] == (0 | (4 << 2)));
+        CHECK(h.xsrc[0] == 0 && h.xsrc[1] == 4 * n3 && h.swords == 4 * (n3 + n4));
+        CHECK(h.soff[0] == 0 && h.soff[4] == 4 * n3 && h.soff[1] == -1 && h.soff[7] == -1);
+        CHECK(h.xchunk.size() == 2);
+        CHECK(h.xchunk[0].src0 == 0 && h.xchunk[0].out0 == 0 && h.xchunk[0].perms == 0);
+        CHECK(h.xchunk[0].code == (1 | (4 << 8) | (3 << 12)) && h.xchunk[0].aoff == (int)a43);
+        CHECK(h.xchunk[1].src0 == 4 * n3 && h.xchunk[1].out0 == 4 * n4);
+        CHECK(h.xchunk[1].code == (1 | (3 << 8) | (4 << 12)) && h.xchunk[1].aoff == 0);
+        CHECK(h.xwords == 4 * (n3 + n4));
+        CHECK(h.xoff[4] == 0 && h.xoff[0] == 4 * n4 && h.xoff[1] == 0 && h.xoff.size() == 8);
     }
     CHECK(run(BB_PDE_MAXWELL, K, deg, toff.data(), tl) == BB_ERR_UNSUPPORTED);
     CHECK(run(BB_PDE_WAVE, 0, deg, toff.data(), tl) == BB_ERR_ARG);
@@ -170,6 +197,11 @@ static void check_batch_setup() {
     missing[11 * 3 + 4] = -1;
     CHECK(run(BB_PDE_WAVE, K, deg, missing.data(), tl) == BB_ERR_ARG);
     CHECK(std::string(ctx.err).find("no transfer table from degree 4 to 3") != std::string::npos);
+    // face (0, 0) names face 1 of element 1, a wall: the faces do not match
+    nf[0] = 1;
+    CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), tl) == BB_ERR_ARG);
+    CHECK(std::string(ctx.err).find("do not match") != std::string::npos);
+    nf[0] = 0;
     // non-finite steps, transfer entries and geometry
     dt[1] = NAN;
     CHECK(run(BB_PDE_WAVE, K, deg, toff.data(), tl) == BB_ERR_NONFINITE);
