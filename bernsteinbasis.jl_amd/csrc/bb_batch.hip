@@ -21,9 +21,14 @@ using namespace bb;
 struct BatchGroup {
     int N = 0;
     int64_t start = 0, count = 0, qoff = 0;  // first element, elements, state offset (doubles)
+    int xc0 = 0, xnc = 0;                    // the transfer-pass chunks of its cross faces
     uint16_t *tab = nullptr;
     float *l0a = nullptr;
     int tiling[2] = {0, 0};
+    // each stage runs the group's transfer pass and stage kernel on a stream of its own,
+    // after the previous stage of every group (fork and join events on the context's stream)
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
 };
 
 struct bb_batch {
@@ -49,6 +54,7 @@ struct bb_batch {
     double *xbuf = nullptr;                   // xwords values: FP64, or FP32 in the first half
     double *send[2] = {nullptr, nullptr};     // swords values each, likewise
     int send_prec = -1;                       // the precision send[cur] holds (-1: none)
+    hipEvent_t fork = nullptr;
     double *q[2] = {nullptr, nullptr};
     int cur = 0;
     double *res64 = nullptr;
@@ -92,22 +98,22 @@ static StageArgs batch_args(bb_batch *b, const BatchGroup &g, int prec, int cur)
     return a;
 }
 
-// r -> m: the node m of a degree-d face triangle whose node in the neighbour's orientation
-// (face permutation p: (a, b, c) of m -> (a2, b2) = its entries perm_entry(p, 0) and
-// perm_entry(p, 1)) is r, d = 1..10
-struct PermInv {
+// m -> r: node m of a degree-d face triangle (d = 1..10) is node r of the same face in the
+// neighbour's orientation, for face permutation p: (a, b, c) of m -> (a2, b2) = its entries
+// perm_entry(p, 0) and perm_entry(p, 1), r = lin2(d, a2, b2)
+struct PermFwd {
     uint8_t v[11][6][kTraceMax];
-    constexpr PermInv() : v{} {
+    constexpr PermFwd() : v{} {
         for (int d = 1; d <= 10; ++d)
             for (int p = 0; p < 6; ++p)
                 for (int b = 0; b <= d; ++b)
                     for (int a = 0; a <= d - b; ++a) {
                         const int x[3] = {a, b, d - a - b};
-                        v[d][p][lin2(d, x[perm_entry(p, 0)], x[perm_entry(p, 1)])] = (uint8_t)lin2(d, a, b);
+                        v[d][p][lin2(d, a, b)] = (uint8_t)lin2(d, x[perm_entry(p, 0)], x[perm_entry(p, 1)]);
                     }
     }
 };
-__device__ constexpr PermInv kPermInv = PermInv();
+__device__ constexpr PermFwd kPermFwd = PermFwd();
 
 // The 16x16x4 matrix-core products: lane L supplies A[L % 16][L / 16] and B[L / 16][L % 16]
 // and holds four entries of column L % 16 of D: rows 4 (L / 16) + e in FP32, rows 4 e + L / 16
@@ -133,14 +139,17 @@ template <> struct XMfma<double> {
 // the neighbour's previous stage) brought to the face's degree N and written in the face's
 // own node order, xbuf[out + m F + f], in the kernels' type T.  A workgroup takes a chunk of
 // at most kXChunk faces of one degree pair, whose traces are one contiguous range:
-//   1. the pair's A tiles (W, Nfp(N) x Nfp(Nn)) and the chunk's traces (to tr[face][j][f],
-//      T) straight into LDS; the k-step padding j in [Nfp(Nn), 4 KS) zeroed;
-//   2. wave w multiplies W by the 16 columns (face 4w + n / 4, field n % 4) of tr on the
-//      matrix cores (16x16x4, FP32 or FP64 accumulation), RT row tiles over the k steps;
-//   3. row r of column (face, f) is written to node kPermInv[N][perm][r] of the face.
+//   1. the pair's A tiles (W, Nfp(N) x Nfp(Nn)) straight into LDS, shared by the 4 waves;
+//   2. wave w takes faces 4w .. 4w + 3, the 16 columns n (face 4w + n / 4, field n % 4) of
+//      the product: each lane loads its B operands of every k step from global memory at
+//      once (row j = 4 ks + L / 16 of column L % 16: 64 contiguous bytes per face and k
+//      step; zero for the k-step padding j >= Nfp(Nn) and for faces past the chunk), then
+//      multiplies on the matrix cores (16x16x4, FP32 or FP64 accumulation), RT row tiles
+//      over the k steps;
+//   3. the products through LDS (over the A tiles) to the chunk's output range, one 16-byte
+//      record per face node: node m of face k is row kPermFwd[N][perm_k][m] of its column.
 // Bounds: the host builds every offset from the arrays it uploads (N, Nn in 1..10); a chunk
 // reads nc Nfp(Nn) F values from src0 and writes nc Nfp(N) F values from out0.
-constexpr int kXFs = 4 * 68 + 16;  // LDS stride of a face's trace: 4 x (66 nodes, padded to 68) + 16
 constexpr int xa_tiles_max() {
     int m = 0;
     for (int N = 1; N <= 10; ++N)
@@ -151,73 +160,77 @@ constexpr int xa_tiles_max() {
     return m;
 }
 constexpr int kXATiles = xa_tiles_max();  // 70: N = 10 <- 9
+constexpr int kXKS = (kTraceMax + 3) / 4;  // k steps at most (17)
 typedef __attribute__((address_space(3))) void lds_void;
+// (at least 7 waves per SIMD: otherwise the compiler hoists every k step's LDS reads of A
+// and the FP32 kernel takes 120 VGPRs, 3 waves per SIMD)
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_xtrace(const XChunk *__restrict__ chunk, int64_t nchunk,
-                                                const void *__restrict__ send, const void *__restrict__ xa,
-                                                void *__restrict__ xbuf) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_xtrace(
+    const XChunk *__restrict__ chunk, int64_t nchunk, const void *__restrict__ send, const void *__restrict__ xa,
+    void *__restrict__ xbuf) {
     using T = std::conditional_t<MIXED, float, double>;
     using M = XMfma<T>;
     constexpr int F = 4;
-    __shared__ __align__(16) T tr[kXChunk * kXFs];
     __shared__ __align__(16) T as[kXATiles * 64];
     const int tid = threadIdx.x, w = tid >> 6, L = tid & 63;
     for (int64_t c = xcd_block(blockIdx.x, gridDim.x); c < nchunk; c += gridDim.x) {
         const XChunk cd = chunk[c];
         const int nc = cd.code & 255, N = (cd.code >> 8) & 15, Nn = (cd.code >> 12) & 15;
         const int nf = nfp2(N), nfn = nfp2(Nn);
-        const int KS = (nfn + 3) / 4, nfnp = 4 * KS, RT = (nf + 15) / 16, NA = RT * KS * 64;
-        __syncthreads();  // the previous chunk's reads of tr and as are done
-        // 16-byte pieces straight into LDS, lane L of a wave to its destination + 16 L
+        const int KS = (nfn + 3) / 4, RT = (nf + 15) / 16, NA = RT * KS * 64;
+        __syncthreads();  // the previous chunk's reads of as are done
         {
+            // 16-byte pieces straight into LDS, lane L of a wave to its destination + 16 L
             const char *ag = reinterpret_cast<const char *>(xa) + (size_t)cd.aoff * sizeof(T);
             const int P = NA * (int)sizeof(T) / 16;
             for (int p0 = 64 * w; p0 < P; p0 += 256)
                 if (p0 + L < P)
                     __builtin_amdgcn_global_load_lds(ag + (size_t)(p0 + L) * 16, (lds_void *)(as + p0 * 16 / sizeof(T)),
                                                      16, 0, 0);
-            // face k's trace: nfn F values (a multiple of 16 bytes) from src0 + k nfn F
-            const char *sg = reinterpret_cast<const char *>(send) + (size_t)cd.src0 * sizeof(T);
-            const int PF = nfn * F * (int)sizeof(T) / 16;
-            for (int k = w; k < nc; k += 4) {
-                const char *sk = sg + (size_t)k * PF * 16;
-                for (int p0 = 0; p0 < PF; p0 += 64)
-                    if (p0 + L < PF)
-                        __builtin_amdgcn_global_load_lds(sk + (size_t)(p0 + L) * 16,
-                                                         (lds_void *)(tr + k * kXFs + p0 * 16 / sizeof(T)), 16, 0, 0);
-            }
         }
-        // the k-step padding of every face (at most 3 nodes of F values)
-        for (int t = tid; t < nc * 12; t += 256) {
-            const int k = t / 12, q = t - k * 12;
-            if (4 * nfn + q < 4 * nfnp) tr[k * kXFs + 4 * nfn + q] = T(0);
-        }
+        // this lane's B operands: face k, field L & 3, rows j = 4 ks + L / 16
+        const int k = 4 * w + ((L & 15) >> 2);
+        const T *src = reinterpret_cast<const T *>(send) + cd.src0 + (int64_t)k * nfn * F + 4 * (L >> 4) + (L & 3);
+        T b[kXKS];
+#pragma unroll
+        for (int ks = 0; ks < kXKS; ++ks) b[ks] = (ks < KS && k < nc && 4 * ks + (L >> 4) < nfn) ? src[16 * ks] : T(0);
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's pieces are in LDS
         __syncthreads();
+        typename M::v4 acc[5];
+#pragma unroll
+        for (int rt = 0; rt < 5; ++rt) acc[rt] = typename M::v4{T(0), T(0), T(0), T(0)};
         if (4 * w < nc) {
             const T *ap = as + L;
-            const T *bp = tr + (4 * w + ((L & 15) >> 2)) * kXFs + 4 * (L >> 4) + (L & 3);
-            typename M::v4 acc[5];
 #pragma unroll
-            for (int rt = 0; rt < 5; ++rt) acc[rt] = typename M::v4{T(0), T(0), T(0), T(0)};
-            for (int ks = 0; ks < KS; ++ks) {
-                const T b = bp[16 * ks];
+            for (int ks = 0; ks < kXKS; ++ks)
+                if (ks < KS) {
 #pragma unroll
-                for (int rt = 0; rt < 5; ++rt)
-                    if (rt < RT) acc[rt] = M::op(ap[(ks * RT + rt) * 64], b, acc[rt]);
-            }
-            const int k = 4 * w + ((L & 15) >> 2);
-            if (k < nc) {
-                const uint8_t *pi = kPermInv.v[N][(cd.perms >> (3 * k)) & 7];
-                T *out = reinterpret_cast<T *>(xbuf) + cd.out0 + (int64_t)k * nf * F + (L & 3);
+                    for (int rt = 0; rt < 5; ++rt)
+                        if (rt < RT) acc[rt] = M::op(ap[(ks * RT + rt) * 64], b[ks], acc[rt]);
+                }
+        }
+        // the products -> LDS (over the A tiles, once every wave is done with them) as
+        // os[face][row][f], rows in the neighbour's orientation; then the chunk's output, one
+        // contiguous range, in 16-byte records: record (face k, node m) <- row
+        // kPermFwd[N][perm_k][m]
+        __syncthreads();
+        T *os = as;
+        if (k < nc) {
 #pragma unroll
-                for (int rt = 0; rt < 5; ++rt)
+            for (int rt = 0; rt < 5; ++rt)
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int r = 16 * rt + M::row(L, e);
-                        if (rt < RT && r < nf) out[pi[r] * F] = acc[rt][e];
-                    }
-            }
+                for (int e = 0; e < 4; ++e) {
+                    const int r = 16 * rt + M::row(L, e);
+                    if (rt < RT && r < nf) os[(k * nf + r) * F + (L & 3)] = acc[rt][e];
+                }
+        }
+        __syncthreads();
+        typedef T v4t __attribute__((ext_vector_type(4)));
+        v4t *out = reinterpret_cast<v4t *>(reinterpret_cast<T *>(xbuf) + cd.out0);
+        for (int t = tid; t < nc * nf; t += 256) {
+            const int kk = t / nf, m = t - kk * nf;
+            const int r = kPermFwd.v[N][(cd.perms >> (3 * kk)) & 7][m];
+            out[t] = *reinterpret_cast<const v4t *>(os + (kk * nf + r) * F);
         }
     }
 }
@@ -264,21 +277,22 @@ static int batch_send_ready(bb_batch *b, int prec) {
     return BB_OK;
 }
 
-// the transfer pass on the traces of q[cur]
-static int batch_xtrace(bb_batch *b, int prec, int cur) {
-    if (!b->nchunk) return BB_OK;
-    const unsigned grid = (unsigned)(b->nchunk < (1 << 22) ? b->nchunk : (1 << 22));
+// the transfer pass on the traces of q[cur], chunks [c0, c0 + nc), on stream st
+static int batch_xtrace(bb_batch *b, int prec, int cur, int64_t c0, int64_t nc, hipStream_t st) {
+    if (nc <= 0) return BB_OK;
+    const unsigned grid = (unsigned)(nc < (1 << 22) ? nc : (1 << 22));
     if (prec == BB_PREC_MIXED)
-        k_xtrace<true><<<grid, 256, 0, b->ctx->stream>>>(b->xchunk, b->nchunk, b->send[cur], b->xa32, b->xbuf);
+        k_xtrace<true><<<grid, 256, 0, st>>>(b->xchunk + c0, nc, b->send[cur], b->xa32, b->xbuf);
     else
-        k_xtrace<false><<<grid, 256, 0, b->ctx->stream>>>(b->xchunk, b->nchunk, b->send[cur], b->xa64, b->xbuf);
+        k_xtrace<false><<<grid, 256, 0, st>>>(b->xchunk + c0, nc, b->send[cur], b->xa64, b->xbuf);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bb_fail(b->ctx, BB_ERR_HIP, "transfer pass launch failed: %s", hipGetErrorString(e));
     return BB_OK;
 }
 
-static int batch_launch(bb_batch *b, const BatchGroup &g, int prec, int tiling, const StageArgs &a) {
-    int rc = batch_fn(prec)(g.N, tiling, a, b->ctx->stream, false);
+static int batch_launch(bb_batch *b, const BatchGroup &g, int prec, int tiling, const StageArgs &a,
+                        hipStream_t st = nullptr) {
+    int rc = batch_fn(prec)(g.N, tiling, a, st ? st : b->ctx->stream, false);
     if (rc == BB_ERR_UNSUPPORTED) return bb_fail(b->ctx, rc, "no fused kernel for N=%d with tiling %d", g.N, tiling);
     if (rc) return bb_fail(b->ctx, rc, "fused stage launch failed: %s", hipGetErrorString(t_launch_err));
     return BB_OK;
@@ -319,21 +333,28 @@ static int batch_prepare(bb_batch *b, int prec) {
     return BB_OK;
 }
 
-// enqueue nsteps steps (capturable): per stage, the transfer pass and one launch per degree
-// group; the send buffer must hold the traces of q[*cur] (batch_send_ready)
+// enqueue nsteps steps (capturable): per stage, each degree group's transfer pass and stage
+// kernel on the group's stream, forked from the context's stream and joined back into it (the
+// groups of one stage are independent: each writes its own state, residual, send slots and
+// received traces); the send buffer must hold the traces of q[*cur] (batch_send_ready)
 static int batch_enqueue(bb_batch *b, int prec, double dt_scale, int nsteps, int *cur) {
     int rc;
+    bb_ctx *ctx = b->ctx;
     for (int st = 0; st < nsteps; ++st)
         for (int stage = 0; stage < 5; ++stage) {
-            if ((rc = batch_xtrace(b, prec, *cur))) return rc;
+            BB_HIP(ctx, hipEventRecord(b->fork, ctx->stream));
             for (auto &g : b->groups) {
+                BB_HIP(ctx, hipStreamWaitEvent(g.stream, b->fork, 0));
+                if ((rc = batch_xtrace(b, prec, *cur, g.xc0, g.xnc, g.stream))) return rc;
                 StageArgs a = batch_args(b, g, prec, *cur);
                 a.ca = kLsrkA[stage];
                 a.cb = kLsrkB[stage];
                 a.dt = dt_scale;
                 a.first = stage == 0;
-                if ((rc = batch_launch(b, g, prec, g.tiling[prec], a))) return rc;
+                if ((rc = batch_launch(b, g, prec, g.tiling[prec], a, g.stream))) return rc;
+                BB_HIP(ctx, hipEventRecord(g.done, g.stream));
             }
+            for (auto &g : b->groups) BB_HIP(ctx, hipStreamWaitEvent(ctx->stream, g.done, 0));
             *cur ^= 1;
         }
     return BB_OK;
@@ -373,6 +394,8 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
         groups[i].start = h.groups[i].start;
         groups[i].count = h.groups[i].count;
         groups[i].qoff = h.groups[i].qoff;
+        groups[i].xc0 = h.xrange[groups[i].N].x;
+        groups[i].xnc = h.xrange[groups[i].N].y;
     }
     bb_batch *b = new (std::nothrow) bb_batch();
     if (!b) return bb_fail(ctx, BB_ERR_NOMEM, "host memory allocation failed");
@@ -429,8 +452,12 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
         return fail(err);
     b->nchunk = (int64_t)nch;
     b->nx = (int64_t)nx;
+    if ((err = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming))) return fail(err);
     for (size_t i = 0; i < b->groups.size(); ++i) {
         BatchGroup &g = b->groups[i];
+        if ((err = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking)) ||
+            (err = hipEventCreateWithFlags(&g.done, hipEventDisableTiming)))
+            return fail(err);
         const std::vector<uint16_t> &tab = tabs[i];
         const std::vector<float> &l0a = l0as[i];
         if ((err = hipMalloc(&g.tab, tab.size() * sizeof(uint16_t))) ||
@@ -451,6 +478,8 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
 int bb_batch_destroy(bb_batch *b) {
     if (!b) return BB_ERR_NULL;
     hipStreamSynchronize(b->ctx->stream);
+    for (auto &g : b->groups)
+        if (g.stream) hipStreamSynchronize(g.stream);
     batch_drop_graphs(b);
     void *ptrs[] = {b->geo,   b->geo32, b->invJ, b->nJ,   b->dte,  b->conn, b->nboff, b->q[0],   b->q[1],
                     b->res64, b->res32, b->xoff, b->soff, b->xchunk, b->xnbq, b->xsrc, b->xcode, b->xa64,
@@ -460,7 +489,10 @@ int bb_batch_destroy(bb_batch *b) {
     for (auto &g : b->groups) {
         if (g.tab) hipFree(g.tab);
         if (g.l0a) hipFree(g.l0a);
+        if (g.done) hipEventDestroy(g.done);
+        if (g.stream) hipStreamDestroy(g.stream);
     }
+    if (b->fork) hipEventDestroy(b->fork);
     delete b;
     return BB_OK;
 }
@@ -503,7 +535,8 @@ static int batch_rhs(bb_batch *b, int prec, double *du_host) {
     int rc = batch_prepare(b, prec);
     if (rc) return rc;
     bb_ctx *ctx = b->ctx;
-    if ((rc = batch_send_ready(b, prec)) || (rc = batch_xtrace(b, prec, b->cur))) return rc;
+    if ((rc = batch_send_ready(b, prec)) || (rc = batch_xtrace(b, prec, b->cur, 0, b->nchunk, ctx->stream)))
+        return rc;
     for (auto &g : b->groups) {
         StageArgs a = batch_args(b, g, prec, b->cur);
         a.q_out = nullptr;
@@ -593,7 +626,8 @@ int bb_batch_autotune(bb_batch *b, int prec, int *tilings) {
     if (rc) return rc;
     bb_ctx *ctx = b->ctx;
     // the traces the group kernels read
-    if ((rc = batch_send_ready(b, prec)) || (rc = batch_xtrace(b, prec, b->cur))) return rc;
+    if ((rc = batch_send_ready(b, prec)) || (rc = batch_xtrace(b, prec, b->cur, 0, b->nchunk, ctx->stream)))
+        return rc;
     for (auto &g : b->groups) {
         StageArgs a = batch_args(b, g, prec, b->cur);  // writes the other buffer only
         a.ca = kLsrkA[1];

@@ -128,9 +128,10 @@ struct BatchHost {
     // The traces between elements of different degree.  A face whose neighbour has another
     // degree (a "cross face") receives the neighbour's trace brought to its own degree: W
     // (Nfp(N) x Nfp(Nn), the transfer matrix of the degree pair N <- Nn) times the trace.
-    //  * The cross faces are ordered by (Nn, neighbour element / kXBlock, N, neighbour state
-    //    offset) -- neighbours close together, for every degree pair -- and cut into chunks of
-    //    at most kXChunk consecutive faces of one pair (XChunk).
+    //  * The cross faces are ordered by (N, Nn, neighbour element / kXBlock, neighbour state
+    //    offset) and cut into chunks of at most kXChunk consecutive faces of one pair
+    //    (XChunk); the chunks of the faces of degree N are xrange[N] = (first, count), so
+    //    that each degree group's pass can run by itself, before that group's stage kernel.
     //  * The sending side: the neighbour's face trace, Nfp(Nn) records of F values in its own
     //    face node order, is written by the neighbour's stage kernel (with its new state) to
     //    the send buffer at soff[4 n + fn] (values; -1: that face sends nothing), the slots
@@ -146,6 +147,7 @@ struct BatchHost {
     //  * Per cross face, for the pass that fills the send buffer from a state (xsend_gather):
     //    the neighbour's state offset xnbq, its send slot xsrc, fn | Nn << 2 (xcode).
     std::vector<XChunk> xchunk;
+    int2 xrange[11] = {};
     std::vector<int64_t> xnbq, xsrc, xoff, soff;
     std::vector<int32_t> xcode;
     int64_t xwords = 0, swords = 0;         // trace and send buffer lengths (values)
@@ -242,7 +244,7 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
         for (int64_t t = 0; t < K * 4; ++t)
             if (!bc[t] && deg[nbr[t]] != deg[t / 4]) cross.push_back(t);
         auto key = [&](int64_t t) {
-            return std::make_tuple(deg[nbr[t]], nbr[t] / kXBlock, deg[t / 4], h.nboff[t], t);
+            return std::make_tuple(deg[t / 4], deg[nbr[t]], nbr[t] / kXBlock, h.nboff[t], t);
         };
         std::sort(cross.begin(), cross.end(), [&](int64_t x, int64_t y) { return key(x) < key(y); });
         h.xoff.assign(K * 4, 0);
@@ -264,10 +266,12 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
             h.soff[snd] = sw;
             h.xoff[t] = w;
             h.xcode[i] = nbr_face[t] | (Nn << 2);
-            // a chunk holds consecutive faces of one (Nn, neighbour block, N)
+            // a chunk holds consecutive faces of one (N, Nn, neighbour block)
             const int64_t p = i ? cross[i - 1] : 0;
             const bool cont = i > 0 && (h.xchunk.back().code & 255) < kXChunk && deg[nbr[p]] == Nn &&
                               nbr[p] / kXBlock == nbr[t] / kXBlock && deg[p / 4] == N;
+            if (!cont && h.xrange[N].y == 0) h.xrange[N].x = (int)h.xchunk.size();
+            if (!cont) ++h.xrange[N].y;
             if (cont) {
                 XChunk &c = h.xchunk.back();
                 c.perms |= (int64_t)perm[t] << (3 * (c.code & 255));

@@ -150,6 +150,8 @@ template <int N, int PDE, bool MIXED, int EB, int LM = LIFT_GATHER, bool MD = fa
     // MFMA tiling of L0: MT row tiles of 16, KT k steps of 4
     static constexpr int MT = (NFP + 15) / 16, KT = (NFP + 3) / 4;
     static constexpr bool valid = NT <= 1024 && lds <= 64 * 1024 && (LM != LIFT_GATHER_MFMA || MIXED);
+    // waves per SIMD the register allocation must allow (1: no bound)
+    static constexpr int WPE = (MD && N == 10) ? 7 : 1;
 };
 
 // L0 couples face nodes at most N+1 apart in linear order (the 7-point stencil reaches the
@@ -250,7 +252,8 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
 }
 
 template <int N, int PDE, bool MIXED, int EB, int LM, bool MD = false>
-__global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_stage(StageArgs A) {
+__global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT),
+                             (StageCfg<N, PDE, MIXED, EB, LM, MD>::WPE)) void k_stage(StageArgs A) {
     using Cfg = StageCfg<N, PDE, MIXED, EB, LM, MD>;
     using T = typename Cfg::T;
     using P = Pde<PDE>;
@@ -338,6 +341,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     double qv[F];
     TR rv[F];
     double invJ = 0.0, dte = 0.0;  // read here: a load issued in the epilogue stalls the workgroup's end
+    int64_t sof[4] = {-1, -1, -1, -1};
     int2 cnv[RF];
     T nb[RF][F];
     // per-element data with a uniform index, as scalar loads issued in the prologue rather
@@ -452,6 +456,12 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
             invJ = A.invJ[eg];
             dte = MD ? A.dt * A.dte[eg] : A.dt;
         }
+        // the send slots of the element's faces (mixed degree, one element per workgroup:
+        // uniform, read here rather than at the end where the stores need them)
+        if constexpr (MD && EB == 1) {
+#pragma unroll
+            for (int fc = 0; fc < 4; ++fc) sof[fc] = A.soff[e0 * 4 + fc];
+        }
         // the connectivity of each item (a scalar load with wave-owned chains) and the uniform
         // per-element data are issued with the state loads, so that their latencies overlap
         // rather than follow one another (mixed-degree kernels: with wave-owned chains only;
@@ -506,6 +516,10 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
     // the loaded values are taken here (in scalar registers), so that the loads are not
     // moved into the later phases that use them
     auto pin_scalar = [](auto &x) { asm volatile("" ::"s"(x)); };
+    if constexpr (MD && EB == 1) {
+#pragma unroll
+        for (int fc = 0; fc < 4; ++fc) pin_scalar(sof[fc]);
+    }
     if constexpr (Cfg::PRE_GEO && !MD) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) pin_scalar(gh[c]);
@@ -888,7 +902,7 @@ __global__ __launch_bounds__((StageCfg<N, PDE, MIXED, EB, LM, MD>::NT)) void k_s
                 // this node is node ix[fc] of face fc where its slab index st[fc] is 0
 #pragma unroll
                 for (int fc = 0; fc < 4; ++fc) {
-                    const int64_t o = A.soff[eg * 4 + fc];
+                    const int64_t o = EB == 1 ? sof[fc] : A.soff[eg * 4 + fc];
                     if (st[fc] == 0 && o >= 0) {
                         T *d = reinterpret_cast<T *>(A.xsend) + o + ix[fc] * F;
 #pragma unroll

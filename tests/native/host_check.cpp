@@ -168,20 +168,22 @@ static void check_batch_setup() {
         CHECK(h.xa64[a43 + 128 + 14 + 16 * 2] == 0.0);                        // column 10: padding
         // conn.x is the neighbour element, as in the single-degree kernels
         CHECK(h.conn[0].x == 1 && h.conn[4].x == 0);
-        // the cross faces, ordered by neighbour degree first: face (1, 0) receives from
-        // element 0 (degree 3) before face (0, 0) receives from element 1 (degree 4); one
-        // chunk per pair.  Send slots in that order: element 0 face 0 sends first (n3 nodes)
-        CHECK(h.xnbq.size() == 2 && h.xnbq[0] == 0 && h.xnbq[1] == 4 * np3(3));
-        CHECK(h.xcode[0] == (0 | (3 << 2)) && h.xcode[1] == (0 | (4 << 2)));
-        CHECK(h.xsrc[0] == 0 && h.xsrc[1] == 4 * n3 && h.swords == 4 * (n3 + n4));
-        CHECK(h.soff[0] == 0 && h.soff[4] == 4 * n3 && h.soff[1] == -1 && h.soff[7] == -1);
+        // the cross faces, ordered by the receiving face's degree first: face (0, 0) (degree 3,
+        // from element 1) before face (1, 0) (degree 4, from element 0); one chunk per pair.
+        // Send slots in that order: element 1 face 0 sends first (n4 nodes)
+        CHECK(h.xnbq.size() == 2 && h.xnbq[0] == 4 * np3(3) && h.xnbq[1] == 0);
+        CHECK(h.xcode[0] == (0 | (4 << 2)) && h.xcode[1] == (0 | (3 << 2)));
+        CHECK(h.xsrc[0] == 0 && h.xsrc[1] == 4 * n4 && h.swords == 4 * (n3 + n4));
+        CHECK(h.soff[4] == 0 && h.soff[0] == 4 * n4 && h.soff[1] == -1 && h.soff[7] == -1);
         CHECK(h.xchunk.size() == 2);
         CHECK(h.xchunk[0].src0 == 0 && h.xchunk[0].out0 == 0 && h.xchunk[0].perms == 0);
-        CHECK(h.xchunk[0].code == (1 | (4 << 8) | (3 << 12)) && h.xchunk[0].aoff == (int)a43);
-        CHECK(h.xchunk[1].src0 == 4 * n3 && h.xchunk[1].out0 == 4 * n4);
-        CHECK(h.xchunk[1].code == (1 | (3 << 8) | (4 << 12)) && h.xchunk[1].aoff == 0);
+        CHECK(h.xchunk[0].code == (1 | (3 << 8) | (4 << 12)) && h.xchunk[0].aoff == 0);
+        CHECK(h.xchunk[1].src0 == 4 * n4 && h.xchunk[1].out0 == 4 * n3);
+        CHECK(h.xchunk[1].code == (1 | (4 << 8) | (3 << 12)) && h.xchunk[1].aoff == (int)a43);
+        CHECK(h.xrange[3].x == 0 && h.xrange[3].y == 1 && h.xrange[4].x == 1 && h.xrange[4].y == 1);
+        CHECK(h.xrange[5].y == 0);
         CHECK(h.xwords == 4 * (n3 + n4));
-        CHECK(h.xoff[4] == 0 && h.xoff[0] == 4 * n4 && h.xoff[1] == 0 && h.xoff.size() == 8);
+        CHECK(h.xoff[0] == 0 && h.xoff[4] == 4 * n3 && h.xoff[1] == 0 && h.xoff.size() == 8);
     }
     CHECK(run(BB_PDE_MAXWELL, K, deg, toff.data(), tl) == BB_ERR_UNSUPPORTED);
     CHECK(run(BB_PDE_WAVE, 0, deg, toff.data(), tl) == BB_ERR_ARG);
