@@ -10,6 +10,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "bb_dispatch.hpp"
@@ -25,11 +26,14 @@ struct BatchGroup {
     uint16_t *tab = nullptr;
     float *l0a = nullptr;
     int tiling[2] = {0, 0};
-    // each stage runs the group's transfer pass and stage kernel on a stream of its own,
-    // after the previous stage of every group (fork and join events on the context's stream)
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    float cost[2] = {0.f, 0.f};  // the autotuned stage kernel's time (ms), 0 before tuning
 };
+
+// Each stage runs the groups' transfer passes and stage kernels on kLanes streams ("lanes"),
+// forked from the context's stream after the previous stage and joined back into it: the
+// groups of one stage are independent (each writes its own state, residual, send slots and
+// received traces), so they overlap, and the lanes are balanced by cost (longest first).
+constexpr int kLanes = 4;
 
 struct bb_batch {
     bb_ctx *ctx = nullptr;
@@ -51,10 +55,13 @@ struct bb_batch {
     int32_t *xcode = nullptr;
     double *xa64 = nullptr;
     float *xa32 = nullptr;
+    uint64_t *xmask = nullptr;                // [121][2] the pairs' nonzero A tiles
     double *xbuf = nullptr;                   // xwords values: FP64, or FP32 in the first half
     double *send[2] = {nullptr, nullptr};     // swords values each, likewise
     int send_prec = -1;                       // the precision send[cur] holds (-1: none)
     hipEvent_t fork = nullptr;
+    hipStream_t lane[kLanes] = {};
+    hipEvent_t lane_done[kLanes] = {};
     double *q[2] = {nullptr, nullptr};
     int cur = 0;
     double *res64 = nullptr;
@@ -74,35 +81,38 @@ static bool batch_valid(int prec, int N, int tiling) {
     return batch_fn(prec)(N, tiling, a, nullptr, true) == BB_OK;
 }
 
-static StageArgs batch_args(bb_batch *b, const BatchGroup &g, int prec, int cur) {
+// the stage arguments of elements [e0, e0 + ne) of group g (ne < 0: the whole group)
+static StageArgs batch_args(bb_batch *b, const BatchGroup &g, int prec, int cur, int64_t e0 = 0, int64_t ne = -1) {
     StageArgs a{};
-    a.q_in = b->q[cur] + g.qoff;
-    a.q_out = b->q[cur ^ 1] + g.qoff;
-    a.res = prec == BB_PREC_MIXED ? (void *)(b->res32 + g.qoff) : (void *)(b->res64 + g.qoff);
-    a.geoh = prec == BB_PREC_MIXED ? (const void *)(b->geo32 + g.start * 9) : (const void *)(b->geo + g.start * 9);
-    a.invJ = b->invJ + g.start;
-    a.nJ = b->nJ + g.start * 20;
-    a.conn = b->conn + g.start * 4;
+    const int64_t s = g.start + e0, qo = g.qoff + e0 * b->F * np3(g.N);
+    a.q_in = b->q[cur] + qo;
+    a.q_out = b->q[cur ^ 1] + qo;
+    a.res = prec == BB_PREC_MIXED ? (void *)(b->res32 + qo) : (void *)(b->res64 + qo);
+    a.geoh = prec == BB_PREC_MIXED ? (const void *)(b->geo32 + s * 9) : (const void *)(b->geo + s * 9);
+    a.invJ = b->invJ + s;
+    a.nJ = b->nJ + s * 20;
+    a.conn = b->conn + s * 4;
     a.tab = g.tab;
     a.l0a = g.l0a;
-    a.K = g.count;
+    a.K = ne < 0 ? g.count : ne;
     a.flux_kind = b->flux_kind;
     for (int f = 0; f < 6; ++f) a.sign[f] = b->sign[f];
     a.q_nb = b->q[cur];
-    a.nboff = b->nboff + g.start * 4;
-    a.dte = b->dte + g.start;
+    a.nboff = b->nboff + s * 4;
+    a.dte = b->dte + s;
     a.xbuf = b->xbuf;
-    a.xoff = b->xoff + g.start * 4;
+    a.xoff = b->xoff + s * 4;
     a.xsend = b->send[cur ^ 1];
-    a.soff = b->soff + g.start * 4;
+    a.soff = b->soff + s * 4;
     return a;
 }
 
 // m -> r: node m of a degree-d face triangle (d = 1..10) is node r of the same face in the
 // neighbour's orientation, for face permutation p: (a, b, c) of m -> (a2, b2) = its entries
 // perm_entry(p, 0) and perm_entry(p, 1), r = lin2(d, a2, b2)
-struct PermFwd {
-    uint8_t v[11][6][kTraceMax];
+constexpr int kPermRow = (kTraceMax + 3) / 4 * 4;  // 68: rows of whole 32-bit words
+struct alignas(16) PermFwd {
+    uint8_t v[11][6][kPermRow];
     constexpr PermFwd() : v{} {
         for (int d = 1; d <= 10; ++d)
             for (int p = 0; p < 6; ++p)
@@ -167,18 +177,23 @@ typedef __attribute__((address_space(3))) void lds_void;
 template <bool MIXED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_xtrace(
     const XChunk *__restrict__ chunk, int64_t nchunk, const void *__restrict__ send, const void *__restrict__ xa,
-    void *__restrict__ xbuf) {
+    const uint64_t *__restrict__ xmask, void *__restrict__ xbuf) {
     using T = std::conditional_t<MIXED, float, double>;
     using M = XMfma<T>;
     constexpr int F = 4;
+    constexpr int PW = 6 * kPermRow / 4;  // the degree's PermFwd rows in 32-bit words
     __shared__ __align__(16) T as[kXATiles * 64];
+    __shared__ uint32_t pfw[PW];
     const int tid = threadIdx.x, w = tid >> 6, L = tid & 63;
     for (int64_t c = xcd_block(blockIdx.x, gridDim.x); c < nchunk; c += gridDim.x) {
         const XChunk cd = chunk[c];
         const int nc = cd.code & 255, N = (cd.code >> 8) & 15, Nn = (cd.code >> 12) & 15;
         const int nf = nfp2(N), nfn = nfp2(Nn);
         const int KS = (nfn + 3) / 4, RT = (nf + 15) / 16, NA = RT * KS * 64;
-        __syncthreads();  // the previous chunk's reads of as are done
+        // the A tiles of the pair that are not all zero: bit ks RT + rt (degree elevations by
+        // one or two have mostly zero tiles)
+        const uint64_t mk0 = xmask[2 * (11 * N + Nn)], mk1 = xmask[2 * (11 * N + Nn) + 1];
+        __syncthreads();  // the previous chunk's reads of as and pfw are done
         {
             // 16-byte pieces straight into LDS, lane L of a wave to its destination + 16 L
             const char *ag = reinterpret_cast<const char *>(xa) + (size_t)cd.aoff * sizeof(T);
@@ -194,6 +209,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
         T b[kXKS];
 #pragma unroll
         for (int ks = 0; ks < kXKS; ++ks) b[ks] = (ks < KS && k < nc && 4 * ks + (L >> 4) < nfn) ? src[16 * ks] : T(0);
+        // the degree's node permutations, for the output
+        uint32_t pv = 0;
+        if (tid < PW) pv = reinterpret_cast<const uint32_t *>(kPermFwd.v[N])[tid];
+        if (tid < PW) pfw[tid] = pv;
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's pieces are in LDS
         __syncthreads();
         typename M::v4 acc[5];
@@ -205,8 +224,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
             for (int ks = 0; ks < kXKS; ++ks)
                 if (ks < KS) {
 #pragma unroll
-                    for (int rt = 0; rt < 5; ++rt)
-                        if (rt < RT) acc[rt] = M::op(ap[(ks * RT + rt) * 64], b[ks], acc[rt]);
+                    for (int rt = 0; rt < 5; ++rt) {
+                        const int bit = ks * RT + rt;  // < 85
+                        if (rt < RT && (((bit < 64 ? mk0 >> bit : mk1 >> (bit - 64)) & 1) != 0))
+                            acc[rt] = M::op(ap[(ks * RT + rt) * 64], b[ks], acc[rt]);
+                    }
                 }
         }
         // the products -> LDS (over the A tiles, once every wave is done with them) as
@@ -227,10 +249,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
         __syncthreads();
         typedef T v4t __attribute__((ext_vector_type(4)));
         v4t *out = reinterpret_cast<v4t *>(reinterpret_cast<T *>(xbuf) + cd.out0);
+        const uint8_t *pb = reinterpret_cast<const uint8_t *>(pfw);
+        // record t = kk nf + m, kk and m stepped along with t
+        int kk = tid / nf, m = tid - kk * nf;
+        const int dk = 256 / nf, dm = 256 - dk * nf;
         for (int t = tid; t < nc * nf; t += 256) {
-            const int kk = t / nf, m = t - kk * nf;
-            const int r = kPermFwd.v[N][(cd.perms >> (3 * kk)) & 7][m];
+            const int r = pb[((cd.perms >> (3 * kk)) & 7) * kPermRow + m];
             out[t] = *reinterpret_cast<const v4t *>(os + (kk * nf + r) * F);
+            kk += dk;
+            m += dm;
+            if (m >= nf) {
+                m -= nf;
+                ++kk;
+            }
         }
     }
 }
@@ -282,9 +313,9 @@ static int batch_xtrace(bb_batch *b, int prec, int cur, int64_t c0, int64_t nc, 
     if (nc <= 0) return BB_OK;
     const unsigned grid = (unsigned)(nc < (1 << 22) ? nc : (1 << 22));
     if (prec == BB_PREC_MIXED)
-        k_xtrace<true><<<grid, 256, 0, st>>>(b->xchunk + c0, nc, b->send[cur], b->xa32, b->xbuf);
+        k_xtrace<true><<<grid, 256, 0, st>>>(b->xchunk + c0, nc, b->send[cur], b->xa32, b->xmask, b->xbuf);
     else
-        k_xtrace<false><<<grid, 256, 0, st>>>(b->xchunk + c0, nc, b->send[cur], b->xa64, b->xbuf);
+        k_xtrace<false><<<grid, 256, 0, st>>>(b->xchunk + c0, nc, b->send[cur], b->xa64, b->xmask, b->xbuf);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bb_fail(b->ctx, BB_ERR_HIP, "transfer pass launch failed: %s", hipGetErrorString(e));
     return BB_OK;
@@ -333,28 +364,56 @@ static int batch_prepare(bb_batch *b, int prec) {
     return BB_OK;
 }
 
+// the lanes' groups in issue order (longest processing time first, each to the least loaded
+// lane); cost: the autotuned time, else the group's state size.  (Cutting the largest group
+// into element ranges on several lanes was slower on the 64-mesh workload, 15.0-15.1 against
+// 14.6 ms per step: concurrent kernels share the chip's bandwidth, and the cut group's parts
+// then hold two lanes for the whole stage.)
+static void batch_schedule(const bb_batch *b, int prec, std::vector<int> (&lanes)[kLanes]) {
+    std::vector<std::pair<double, int>> c;
+    for (size_t i = 0; i < b->groups.size(); ++i) {
+        const BatchGroup &g = b->groups[i];
+        c.push_back({g.cost[prec] > 0.f ? (double)g.cost[prec] : 1e-9 * (double)g.count * np3(g.N), (int)i});
+    }
+    std::stable_sort(c.begin(), c.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    double load[kLanes] = {};
+    for (auto &l : lanes) l.clear();
+    for (const auto &ci : c) {
+        int m = 0;
+        for (int l = 1; l < kLanes; ++l)
+            if (load[l] < load[m]) m = l;
+        load[m] += ci.first;
+        lanes[m].push_back(ci.second);
+    }
+}
+
 // enqueue nsteps steps (capturable): per stage, each degree group's transfer pass and stage
-// kernel on the group's stream, forked from the context's stream and joined back into it (the
-// groups of one stage are independent: each writes its own state, residual, send slots and
-// received traces); the send buffer must hold the traces of q[*cur] (batch_send_ready)
+// kernel on its lane; the send buffer must hold the traces of q[*cur] (batch_send_ready)
 static int batch_enqueue(bb_batch *b, int prec, double dt_scale, int nsteps, int *cur) {
     int rc;
     bb_ctx *ctx = b->ctx;
+    std::vector<int> lanes[kLanes];
+    batch_schedule(b, prec, lanes);
     for (int st = 0; st < nsteps; ++st)
         for (int stage = 0; stage < 5; ++stage) {
             BB_HIP(ctx, hipEventRecord(b->fork, ctx->stream));
-            for (auto &g : b->groups) {
-                BB_HIP(ctx, hipStreamWaitEvent(g.stream, b->fork, 0));
-                if ((rc = batch_xtrace(b, prec, *cur, g.xc0, g.xnc, g.stream))) return rc;
-                StageArgs a = batch_args(b, g, prec, *cur);
-                a.ca = kLsrkA[stage];
-                a.cb = kLsrkB[stage];
-                a.dt = dt_scale;
-                a.first = stage == 0;
-                if ((rc = batch_launch(b, g, prec, g.tiling[prec], a, g.stream))) return rc;
-                BB_HIP(ctx, hipEventRecord(g.done, g.stream));
+            for (int l = 0; l < kLanes; ++l) {
+                if (lanes[l].empty()) continue;
+                BB_HIP(ctx, hipStreamWaitEvent(b->lane[l], b->fork, 0));
+                for (int gi : lanes[l]) {
+                    const BatchGroup &g = b->groups[gi];
+                    if ((rc = batch_xtrace(b, prec, *cur, g.xc0, g.xnc, b->lane[l]))) return rc;
+                    StageArgs a = batch_args(b, g, prec, *cur);
+                    a.ca = kLsrkA[stage];
+                    a.cb = kLsrkB[stage];
+                    a.dt = dt_scale;
+                    a.first = stage == 0;
+                    if ((rc = batch_launch(b, g, prec, g.tiling[prec], a, b->lane[l]))) return rc;
+                }
+                BB_HIP(ctx, hipEventRecord(b->lane_done[l], b->lane[l]));
             }
-            for (auto &g : b->groups) BB_HIP(ctx, hipStreamWaitEvent(ctx->stream, g.done, 0));
+            for (int l = 0; l < kLanes; ++l)
+                if (!lanes[l].empty()) BB_HIP(ctx, hipStreamWaitEvent(ctx->stream, b->lane_done[l], 0));
             *cur ^= 1;
         }
     return BB_OK;
@@ -431,6 +490,7 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
         (err = hipMalloc(&b->send[1], (h.swords + 1) * sizeof(double))) ||
         (err = hipMalloc(&b->xa64, (h.xa64.size() + 1) * sizeof(double))) ||
         (err = hipMalloc(&b->xa32, (h.xa32.size() + 1) * sizeof(float))) ||
+        (err = hipMalloc(&b->xmask, h.xmask.size() * sizeof(uint64_t))) ||
         (err = hipMalloc(&b->xbuf, (h.xwords + 1) * sizeof(double))) ||
         (err = hipMalloc(&b->q[1], words * sizeof(double))))
         return fail(err);
@@ -448,16 +508,18 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
         (err = hipMemcpy(b->xsrc, h.xsrc.data(), nx * sizeof(int64_t), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->xcode, h.xcode.data(), nx * sizeof(int32_t), hipMemcpyHostToDevice)) ||
         (err = hipMemcpy(b->xa64, h.xa64.data(), h.xa64.size() * sizeof(double), hipMemcpyHostToDevice)) ||
-        (err = hipMemcpy(b->xa32, h.xa32.data(), h.xa32.size() * sizeof(float), hipMemcpyHostToDevice)))
+        (err = hipMemcpy(b->xa32, h.xa32.data(), h.xa32.size() * sizeof(float), hipMemcpyHostToDevice)) ||
+        (err = hipMemcpy(b->xmask, h.xmask.data(), h.xmask.size() * sizeof(uint64_t), hipMemcpyHostToDevice)))
         return fail(err);
     b->nchunk = (int64_t)nch;
     b->nx = (int64_t)nx;
     if ((err = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming))) return fail(err);
+    for (int l = 0; l < kLanes; ++l)
+        if ((err = hipStreamCreateWithFlags(&b->lane[l], hipStreamNonBlocking)) ||
+            (err = hipEventCreateWithFlags(&b->lane_done[l], hipEventDisableTiming)))
+            return fail(err);
     for (size_t i = 0; i < b->groups.size(); ++i) {
         BatchGroup &g = b->groups[i];
-        if ((err = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking)) ||
-            (err = hipEventCreateWithFlags(&g.done, hipEventDisableTiming)))
-            return fail(err);
         const std::vector<uint16_t> &tab = tabs[i];
         const std::vector<float> &l0a = l0as[i];
         if ((err = hipMalloc(&g.tab, tab.size() * sizeof(uint16_t))) ||
@@ -478,19 +540,21 @@ static int batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const in
 int bb_batch_destroy(bb_batch *b) {
     if (!b) return BB_ERR_NULL;
     hipStreamSynchronize(b->ctx->stream);
-    for (auto &g : b->groups)
-        if (g.stream) hipStreamSynchronize(g.stream);
+    for (auto &l : b->lane)
+        if (l) hipStreamSynchronize(l);
     batch_drop_graphs(b);
     void *ptrs[] = {b->geo,   b->geo32, b->invJ, b->nJ,   b->dte,  b->conn, b->nboff, b->q[0],   b->q[1],
                     b->res64, b->res32, b->xoff, b->soff, b->xchunk, b->xnbq, b->xsrc, b->xcode, b->xa64,
-                    b->xa32,  b->xbuf,  b->send[0], b->send[1]};
+                    b->xa32,  b->xmask, b->xbuf,  b->send[0], b->send[1]};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &g : b->groups) {
         if (g.tab) hipFree(g.tab);
         if (g.l0a) hipFree(g.l0a);
-        if (g.done) hipEventDestroy(g.done);
-        if (g.stream) hipStreamDestroy(g.stream);
+    }
+    for (int l = 0; l < kLanes; ++l) {
+        if (b->lane_done[l]) hipEventDestroy(b->lane_done[l]);
+        if (b->lane[l]) hipStreamDestroy(b->lane[l]);
     }
     if (b->fork) hipEventDestroy(b->fork);
     delete b;
@@ -614,6 +678,7 @@ int bb_batch_set_tuning(bb_batch *b, int prec, int N, int tiling) {
     for (auto &g : b->groups)
         if (g.N == N) {
             g.tiling[prec] = tiling;
+            g.cost[prec] = 0.f;
             batch_drop_graphs(b);
             return BB_OK;
         }
@@ -655,6 +720,7 @@ int bb_batch_autotune(bb_batch *b, int prec, int *tilings) {
             }
         if (!best_t) return bb_fail(ctx, BB_ERR_UNSUPPORTED, "no fused kernel compiled for N=%d", g.N);
         g.tiling[prec] = best_t;
+        g.cost[prec] = best;
     }
     batch_drop_graphs(b);
     if (tilings) {

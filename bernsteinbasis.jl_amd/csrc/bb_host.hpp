@@ -153,6 +153,7 @@ struct BatchHost {
     int64_t xwords = 0, swords = 0;         // trace and send buffer lengths (values)
     std::vector<double> xa64;
     std::vector<float> xa32;
+    std::vector<uint64_t> xmask;            // [121][2]: bit ks RT + rt of pair 11 N + Nn: tile not all zero
     std::vector<double> nj5;                // [K][4][5]
     std::vector<double> invJ, geoh;         // [K], [K][9] geo / 2
     std::vector<float> geoh32;              // [K][9]
@@ -222,6 +223,7 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
     }
     // every degree pair present, in the A-operand layout
     std::vector<int64_t> xaoff(121, -1);
+    h.xmask.assign(242, 0);
     for (int64_t t = 0; t < K * 4; ++t) {
         const int N = deg[t / 4], Nn = deg[nbr[t]];
         if (bc[t] || Nn == N || xaoff[11 * N + Nn] >= 0) continue;
@@ -234,6 +236,10 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
                 for (int L = 0; L < 64; ++L) {
                     const int r = 16 * rt + L % 16, c = 4 * ks + L / 16;
                     if (r < nf && c < nfn) h.xa64[a0 + ((int64_t)ks * RT + rt) * 64 + L] = transfer[o + (int64_t)r * nfn + c];
+                    if (h.xa64[a0 + ((int64_t)ks * RT + rt) * 64 + L] != 0.0) {
+                        const int bit = ks * RT + rt;
+                        h.xmask[2 * (11 * N + Nn) + bit / 64] |= uint64_t(1) << (bit % 64);
+                    }
                 }
     }
     if ((int64_t)h.xa64.size() > INT32_MAX) return bb_fail(ctx, BB_ERR_ARG, "transfer tables too large");

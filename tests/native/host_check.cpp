@@ -166,6 +166,9 @@ static void check_batch_setup() {
         CHECK(h.xa64[a43 + 64 + 3 + 16 * 2] == tr[(size_t)3 * n3 + 6]);       // W[3][6]: ks 1, lane 35
         CHECK(h.xa64[a43 + 128 + 14 + 16 * 1] == tr[(size_t)14 * n3 + 9]);    // W[14][9]: ks 2
         CHECK(h.xa64[a43 + 128 + 14 + 16 * 2] == 0.0);                        // column 10: padding
+        // tiles not all zero: every one of the dense projection, the 3 k steps of the elevation
+        CHECK(h.xmask.size() == 242 && h.xmask[2 * (11 * 3 + 4)] == 0xF && h.xmask[2 * (11 * 3 + 4) + 1] == 0);
+        CHECK(h.xmask[2 * (11 * 4 + 3)] == 0x7 && h.xmask[0] == 0);
         // conn.x is the neighbour element, as in the single-degree kernels
         CHECK(h.conn[0].x == 1 && h.conn[4].x == 0);
         // the cross faces, ordered by the receiving face's degree first: face (0, 0) (degree 3,

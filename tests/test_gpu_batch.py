@@ -115,3 +115,45 @@ def test_single_degree_batch_equals_solver(ctx):
     ref_dev = np.concatenate([ref[b.order[e]].T.reshape(-1) for e in range(b.K)])
     assert rel(got, ref_dev) <= 1e-13
     s.close()
+
+
+def test_batch_steps_on_lanes(ctx, orc):
+    """a batch of 3072 elements in four degree groups, stepped on several streams: eager steps
+    and graph replays against the oracle, mixed and FP64, and the traces of a new state after
+    a run in the other precision"""
+    from bbdg.batch import BatchSolver
+
+    b, q = make([8], (1, 4), seed=21)
+    assert b.K == 3072
+    ba = orc.BatchArrays(b, [1, 1, 1, 1])
+    qr = b.to_node_major(q)
+    for _ in range(2):
+        orc.lsrk_step_md(0, qr, ba, nthreads=4)
+    ref = b.from_node_major(qr)
+    s = BatchSolver(b)
+    try:
+        s.set_state(q)
+        s.step(2, "mixed")
+        assert rel(s.get_state(), ref) <= 1e-5
+        s.set_state(q)
+        s.step(2, "f64")
+        eager = s.get_state()
+        assert rel(eager, ref) <= TOL64, rel(eager, ref)
+        s.set_state(q)
+        s.capture(1, "f64")
+        for _ in range(2):
+            s.launch()
+        assert rel(s.get_state(), eager) <= TOL64
+        # a mixed run, then FP64 steps from a new state: the FP64 traces are gathered again
+        s.set_state(q)
+        s.step(1, "mixed")
+        s.set_state(q)
+        s.step(2, "f64")
+        assert rel(s.get_state(), eager) <= TOL64
+        t = s.autotune("mixed")
+        assert set(t) == set(b.Ns)
+        s.set_state(q)
+        s.step(2, "mixed")
+        assert rel(s.get_state(), ref) <= 1e-5
+    finally:
+        s.close()
