@@ -927,7 +927,7 @@ int stage_launch_wave_md_mixed(int N, int tiling, const StageArgs &a, hipStream_
 
 constexpr int kStageMaxN = 10;
 // candidate tilings (elements per block, lift organisation), in autotuning order
-constexpr int kStageEB[] = {1, 2, 4};
+constexpr int kStageEB[] = {1, 2, 3, 4};  // 3: mixed-degree kernels (a full wave of degree-3 nodes)
 constexpr int kStageLM[] = {LIFT_GATHER, LIFT_PINGPONG, LIFT_GATHER_MFMA, LIFT_WAVE, LIFT_WAVE_GT, LIFT_WAVE1};
 
 // tiling code: elements per block + 16 * lift organisation
@@ -1009,9 +1009,11 @@ constexpr int tiling_code(int eb, int lm) { return eb + 16 * lm; }
             using W5 = std::integral_constant<int, LIFT_WAVE1>;                                        \
             one(std::integral_constant<int, 1>{}, G0{});                                               \
             one(std::integral_constant<int, 2>{}, G0{});                                               \
+            one(std::integral_constant<int, 3>{}, G0{});                                               \
             one(std::integral_constant<int, 4>{}, G0{});                                               \
             one(std::integral_constant<int, 1>{}, P1{});                                               \
             one(std::integral_constant<int, 2>{}, P1{});                                               \
+            one(std::integral_constant<int, 3>{}, P1{});                                               \
             one(std::integral_constant<int, 4>{}, P1{});                                               \
             one(std::integral_constant<int, 1>{}, W3{});                                               \
             one(std::integral_constant<int, 1>{}, W4{});                                               \

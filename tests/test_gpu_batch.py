@@ -83,7 +83,7 @@ def test_batch_every_tiling(ctx, orc):
     tried = 0
     for N in b.Ns:
         for lift in ("gather", "pingpong", "wave", "wave_gt", "wave1"):
-            for eb in (1, 2, 4):
+            for eb in (1, 2, 3, 4):
                 try:
                     s.set_tuning("f64", N, bbdg.Solver.tiling_code(eb, lift))
                 except bbdg.BBError:
