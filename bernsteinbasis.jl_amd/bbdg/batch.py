@@ -114,8 +114,9 @@ class Batch:
 
 
 class BatchSolver:
-    """Device-resident batch (bb_batch_* of the C ABI): fused stage kernels, one launch per
-    degree group and stage, hipGraph-captured steps, per-group autotuned tilings."""
+    """Device-resident batch (bb_batch_* of the C ABI): per degree group and stage, a transfer
+    pass for the faces whose neighbour has another degree and one fused stage kernel, the
+    groups on four streams; hipGraph-captured steps, per-group autotuned tilings."""
 
     def __init__(self, batch: Batch, pde="wave", flux="upwind", bc_sign=None, ctx: Context | None = None):
         self.ctx = ctx or default_context()
