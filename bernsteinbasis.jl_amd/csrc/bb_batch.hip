@@ -1,11 +1,13 @@
 // bb_batch.hip -- many independent meshes whose elements carry their own degree, stepped
 // together: the elements are stored grouped by degree (group N: all elements of degree N,
-// state [K_N][F][Np(N)] FP64, groups concatenated).  Every LSRK45 stage is one launch of the
-// transfer pass (k_xtrace: the face trace of every neighbour of another degree, brought to
-// the element's degree with a host-built transfer matrix -- degree elevation from below, L2
-// projection from above -- and written in the element's face node order), then one launch
-// of the fused stage kernel per degree present, all reading the previous stage's state of
-// every group (ping-pong buffers) and writing the next.  Steps are captured into hipGraphs;
+// state [K_N][F][Np(N)] FP64, groups concatenated).  Every LSRK45 stage runs, per degree
+// present, the transfer pass (k_xtrace: for the group's faces whose neighbour has another
+// degree, the neighbour's face trace -- written to a send slot by the neighbour's previous
+// stage -- brought to the group's degree by a host-built transfer matrix on the matrix cores:
+// degree elevation from below, L2 projection from above) and then the group's fused stage
+// kernel, which reads the transferred traces like same-degree neighbour traces and writes its
+// own outgoing traces with its new state.  The groups of a stage run on four streams and all
+// read the previous stage's state (ping-pong buffers).  Steps are captured into hipGraphs;
 // the tiling of each degree group is autotuned separately.
 #include <math.h>
 #include <string.h>
