@@ -109,22 +109,6 @@ static StageArgs batch_args(bb_batch *b, const BatchGroup &g, int prec, int cur,
     return a;
 }
 
-// m -> r: node m of a degree-d face triangle (d = 1..10) is node r of the same face in the
-// neighbour's orientation, for face permutation p: (a, b, c) of m -> (a2, b2) = its entries
-// perm_entry(p, 0) and perm_entry(p, 1), r = lin2(d, a2, b2)
-constexpr int kPermRow = (kTraceMax + 3) / 4 * 4;  // 68: rows of whole 32-bit words
-struct alignas(16) PermFwd {
-    uint8_t v[11][6][kPermRow];
-    constexpr PermFwd() : v{} {
-        for (int d = 1; d <= 10; ++d)
-            for (int p = 0; p < 6; ++p)
-                for (int b = 0; b <= d; ++b)
-                    for (int a = 0; a <= d - b; ++a) {
-                        const int x[3] = {a, b, d - a - b};
-                        v[d][p][lin2(d, a, b)] = (uint8_t)lin2(d, x[perm_entry(p, 0)], x[perm_entry(p, 1)]);
-                    }
-    }
-};
 __device__ constexpr PermFwd kPermFwd = PermFwd();
 
 // The 16x16x4 matrix-core products: lane L supplies A[L % 16][L / 16] and B[L / 16][L % 16]

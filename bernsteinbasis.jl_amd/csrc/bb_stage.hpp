@@ -71,20 +71,22 @@ constexpr double kLsrkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0
 
 constexpr int kTraceMax = 66;  // face nodes of the highest degree of a fused kernel (N = 10)
 
-// v[d][f][m]: volume index of node m (linear order of the face triangle) of face f of a
-// degree-d element, d = 1..10 -- the face traces of neighbours of any degree, as the
-// mixed-degree kernels stage them (a table read in place of decoding (a,b) from m and
-// composing the volume index at a run-time degree)
-struct FaceVidAll {
-    uint16_t v[11][4][kTraceMax];
-    constexpr FaceVidAll() : v{} {
+// m -> r: node m of a degree-d face triangle (d = 1..10) is node r of the same face in the
+// neighbour's orientation, for face permutation p: (a, b, c) of m -> (a2, b2) = its entries
+// perm_entry(p, 0) and perm_entry(p, 1), r = lin2(d, a2, b2)
+constexpr int kPermRow = (kTraceMax + 3) / 4 * 4;  // 68: rows of whole 32-bit words
+struct alignas(16) PermFwd {
+    uint8_t v[11][6][kPermRow];
+    constexpr PermFwd() : v{} {
         for (int d = 1; d <= 10; ++d)
-            for (int f = 0; f < 4; ++f)
+            for (int p = 0; p < 6; ++p)
                 for (int b = 0; b <= d; ++b)
-                    for (int a = 0; a <= d - b; ++a) v[d][f][lin2(d, a, b)] = (uint16_t)face_vidx(d, f, a, b);
+                    for (int a = 0; a <= d - b; ++a) {
+                        const int x[3] = {a, b, d - a - b};
+                        v[d][p][lin2(d, a, b)] = (uint8_t)lin2(d, x[perm_entry(p, 0)], x[perm_entry(p, 1)]);
+                    }
     }
 };
-__device__ constexpr FaceVidAll kFaceVidAll = FaceVidAll();
 
 // Lift organisation inside the stage kernel:
 //  GATHER   : every chain vector of every face is kept in LDS (slab s of face f at the

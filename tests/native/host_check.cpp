@@ -47,17 +47,29 @@ static void check_tables() {
     }
 }
 
-// the mixed-degree kernels' table of every degree's face-node volume indices against the
-// run-time decode it replaces
-static void check_face_table() {
-    static const FaceVidAll t;
-    for (int d = 1; d <= kStageMaxN; ++d)
-        for (int f = 0; f < 4; ++f)
+// the transfer pass's node permutations (node m of a face -> its node in the neighbour's
+// orientation) against the single-degree kernels' neighbour-node table: the same volume
+// index of the neighbour, and a bijection of the face's nodes
+static void check_face_perm() {
+    static const PermFwd t;
+    for (int d = 1; d <= kStageMaxN; ++d) {
+        const std::vector<uint16_t> tab = stage_tables(d);
+        const uint16_t *nvid = tab.data() + 2 * np3(d) + 4 * nfp2(d);
+        for (int p = 0; p < 6; ++p) {
+            std::vector<int> seen(nfp2(d), 0);
             for (int m = 0; m < nfp2(d); ++m) {
+                const int r = t.v[d][p][m];
+                CHECK(r < nfp2(d));
+                if (r >= nfp2(d)) continue;
+                ++seen[r];
+                if (p == 0) CHECK(r == m);
                 int a, b;
-                decode2(d, m, a, b);
-                CHECK(t.v[d][f][m] == face_vidx(d, f, a, b) && t.v[d][f][m] < np3(d));
+                decode2(d, r, a, b);
+                for (int g = 0; g < 4; ++g) CHECK(face_vidx(d, g, a, b) == nvid[(g * 6 + p) * nfp2(d) + m]);
             }
+            for (int r = 0; r < nfp2(d); ++r) CHECK(seen[r] == 1);
+        }
+    }
 }
 
 static void check_solver_setup() {
@@ -264,7 +276,7 @@ static void check_guard_and_errors() {
 
 int main() {
     check_tables();
-    check_face_table();
+    check_face_perm();
     check_solver_setup();
     check_batch_setup();
     check_guard_and_errors();
