@@ -158,10 +158,12 @@ constexpr int xa_tiles_max() {
 constexpr int kXATiles = xa_tiles_max();  // 70: N = 10 <- 9
 constexpr int kXKS = (kTraceMax + 3) / 4;  // k steps at most (17)
 typedef __attribute__((address_space(3))) void lds_void;
-// (at least 7 waves per SIMD: otherwise the compiler hoists every k step's LDS reads of A
-// and the FP32 kernel takes 120 VGPRs, 3 waves per SIMD)
+// Register bound: 8 waves per SIMD for FP32 (64 VGPRs, no spills; the A tiles' LDS admits 8
+// workgroups per CU; at 7 waves the mixed batch steps were ~1% slower), 4 for FP64, whose
+// A tiles admit 4 workgroups.  Unbounded, the compiler hoists every k step's LDS reads of A
+// and the FP32 kernel takes 120 VGPRs, 3 waves per SIMD.
 template <bool MIXED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_xtrace(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 8 : 4))) void k_xtrace(
     const XChunk *__restrict__ chunk, int64_t nchunk, const void *__restrict__ send, const void *__restrict__ xa,
     const uint64_t *__restrict__ xmask, void *__restrict__ xbuf) {
     using T = std::conditional_t<MIXED, float, double>;

@@ -104,7 +104,6 @@ inline int solver_host_setup(bb_ctx *ctx, int pde, int flux_kind, int N, int64_t
 }
 
 constexpr int kXChunk = 16;  // cross faces of one degree pair per transfer-pass workgroup
-constexpr int kXBlock = 64;  // neighbour elements whose cross faces are ordered together
 
 // a chunk of the transfer pass: its faces' traces at send[src0 ..] (Nfp(Nn) F values per
 // face), results to xbuf[out0 ..] (Nfp(N) F values per face), face k's perm in bits 3k..3k+2
@@ -128,9 +127,9 @@ struct BatchHost {
     // The traces between elements of different degree.  A face whose neighbour has another
     // degree (a "cross face") receives the neighbour's trace brought to its own degree: W
     // (Nfp(N) x Nfp(Nn), the transfer matrix of the degree pair N <- Nn) times the trace.
-    //  * The cross faces are ordered by (N, Nn, neighbour element / kXBlock, neighbour state
-    //    offset) and cut into chunks of at most kXChunk consecutive faces of one pair
-    //    (XChunk); the chunks of the faces of degree N are xrange[N] = (first, count), so
+    //  * The cross faces are ordered by (N, Nn, neighbour state offset) and cut into chunks
+    //    of kXChunk consecutive faces of one pair (XChunk; the last chunk of a pair may hold
+    //    fewer); the chunks of the faces of degree N are xrange[N] = (first, count), so
     //    that each degree group's pass can run by itself, before that group's stage kernel.
     //  * The sending side: the neighbour's face trace, Nfp(Nn) records of F values in its own
     //    face node order, is written by the neighbour's stage kernel (with its new state) to
@@ -249,9 +248,7 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
         std::vector<int64_t> cross;
         for (int64_t t = 0; t < K * 4; ++t)
             if (!bc[t] && deg[nbr[t]] != deg[t / 4]) cross.push_back(t);
-        auto key = [&](int64_t t) {
-            return std::make_tuple(deg[t / 4], deg[nbr[t]], nbr[t] / kXBlock, h.nboff[t], t);
-        };
+        auto key = [&](int64_t t) { return std::make_tuple(deg[t / 4], deg[nbr[t]], h.nboff[t], t); };
         std::sort(cross.begin(), cross.end(), [&](int64_t x, int64_t y) { return key(x) < key(y); });
         h.xoff.assign(K * 4, 0);
         h.soff.assign(K * 4, -1);
@@ -272,10 +269,10 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
             h.soff[snd] = sw;
             h.xoff[t] = w;
             h.xcode[i] = nbr_face[t] | (Nn << 2);
-            // a chunk holds consecutive faces of one (N, Nn, neighbour block)
+            // a chunk holds consecutive faces of one pair (N, Nn)
             const int64_t p = i ? cross[i - 1] : 0;
-            const bool cont = i > 0 && (h.xchunk.back().code & 255) < kXChunk && deg[nbr[p]] == Nn &&
-                              nbr[p] / kXBlock == nbr[t] / kXBlock && deg[p / 4] == N;
+            const bool cont =
+                i > 0 && (h.xchunk.back().code & 255) < kXChunk && deg[nbr[p]] == Nn && deg[p / 4] == N;
             if (!cont && h.xrange[N].y == 0) h.xrange[N].x = (int)h.xchunk.size();
             if (!cont) ++h.xrange[N].y;
             if (cont) {
