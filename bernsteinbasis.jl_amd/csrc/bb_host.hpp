@@ -129,8 +129,7 @@ struct BatchHost {
     // (Nfp(N) x Nfp(Nn), the transfer matrix of the degree pair N <- Nn) times the trace.
     //  * The cross faces are ordered by (N, Nn, neighbour state offset) and cut into chunks
     //    of kXChunk consecutive faces of one pair (XChunk; the last chunk of a pair may hold
-    //    fewer); the chunks of the faces of degree N are xrange[N] = (first, count), so
-    //    that each degree group's pass can run by itself, before that group's stage kernel.
+    //    fewer); one pass runs every chunk.
     //  * The sending side: the neighbour's face trace, Nfp(Nn) records of F values in its own
     //    face node order, is written by the neighbour's stage kernel (with its new state) to
     //    the send buffer at soff[4 n + fn] (values; -1: that face sends nothing), the slots
@@ -146,7 +145,6 @@ struct BatchHost {
     //  * Per cross face, for the pass that fills the send buffer from a state (xsend_gather):
     //    the neighbour's state offset xnbq, its send slot xsrc, fn | Nn << 2 (xcode).
     std::vector<XChunk> xchunk;
-    int2 xrange[11] = {};
     std::vector<int64_t> xnbq, xsrc, xoff, soff;
     std::vector<int32_t> xcode;
     int64_t xwords = 0, swords = 0;         // trace and send buffer lengths (values)
@@ -273,8 +271,6 @@ inline int batch_host_setup(bb_ctx *ctx, int pde, int flux_kind, int64_t K, cons
             const int64_t p = i ? cross[i - 1] : 0;
             const bool cont =
                 i > 0 && (h.xchunk.back().code & 255) < kXChunk && deg[nbr[p]] == Nn && deg[p / 4] == N;
-            if (!cont && h.xrange[N].y == 0) h.xrange[N].x = (int)h.xchunk.size();
-            if (!cont) ++h.xrange[N].y;
             if (cont) {
                 XChunk &c = h.xchunk.back();
                 c.perms |= (int64_t)perm[t] << (3 * (c.code & 255));

@@ -195,8 +195,6 @@ static void check_batch_setup() {
         CHECK(h.xchunk[0].code == (1 | (3 << 8) | (4 << 12)) && h.xchunk[0].aoff == 0);
         CHECK(h.xchunk[1].src0 == 4 * n4 && h.xchunk[1].out0 == 4 * n3);
         CHECK(h.xchunk[1].code == (1 | (4 << 8) | (3 << 12)) && h.xchunk[1].aoff == (int)a43);
-        CHECK(h.xrange[3].x == 0 && h.xrange[3].y == 1 && h.xrange[4].x == 1 && h.xrange[4].y == 1);
-        CHECK(h.xrange[5].y == 0);
         CHECK(h.xwords == 4 * (n3 + n4));
         CHECK(h.xoff[0] == 0 && h.xoff[4] == 4 * n3 && h.xoff[1] == 0 && h.xoff.size() == 8);
     }
