@@ -382,9 +382,9 @@ def batch_bench(ctx, steps=100, n_inst=64, seed=5):
     periodic Kuhn mesh of n^3 x 6 tets with n ~ U{4..16}; every element has its own
     degree N ~ U{3..10}; Gaussian pulses p = exp(-|x - c|^2 / w^2) with c ~ U[0.3,0.7]^3,
     w ~ U[0.05,0.2], u = v = w = 0 (all from seed 5); per-instance step 0.5 h / 10^2.
-    Elements are grouped by degree; every stage runs, per degree group, the transfer pass of
-    the group's faces whose neighbour has another degree and the fused stage kernel, the groups
-    on four streams; a step is one hipGraph replay, tilings are autotuned per group.  Timed: steps
+    Elements are grouped by degree; every stage runs the transfer pass of the faces whose
+    neighbour has another degree, then each degree group's fused stage kernel, the groups on
+    four streams; a step is one hipGraph replay, tilings are autotuned per group.  Timed: steps
     LSRK45 steps in mixed precision and in FP64; errors of mixed against FP64 after them."""
     from bbdg import mesh
     from bbdg.batch import Batch, BatchSolver

@@ -114,8 +114,8 @@ class Batch:
 
 
 class BatchSolver:
-    """Device-resident batch (bb_batch_* of the C ABI): per degree group and stage, a transfer
-    pass for the faces whose neighbour has another degree and one fused stage kernel, the
+    """Device-resident batch (bb_batch_* of the C ABI): per stage, a transfer pass for the
+    faces whose neighbour has another degree, then one fused stage kernel per degree group, the
     groups on four streams; hipGraph-captured steps, per-group autotuned tilings."""
 
     def __init__(self, batch: Batch, pde="wave", flux="upwind", bc_sign=None, ctx: Context | None = None):

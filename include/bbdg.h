@@ -257,9 +257,9 @@ int bb_rhs_combine(bb_ctx *ctx, int pde, int N, int64_t K, double *du, const dou
  * row-major matrix at transfer + transfer_off[11 N + Nn] (degree elevation for Nn < N,
  * the L2 projection for Nn > N), then the upwind flux is that of bb_solver_create.
  * Faces must match mutually (the neighbour across a face names that face back).
- * Each stage launches, per degree present, a transfer pass for the faces of that degree
- * whose neighbour has another degree and the fused stage kernel, on four streams joined
- * into the context's stream (also inside captured graphs). */
+ * Each stage launches a transfer pass for the faces whose neighbour has another degree,
+ * then the fused stage kernel of each degree present, those on four streams joined into
+ * the context's stream (also inside captured graphs). */
 int bb_batch_create(bb_ctx *ctx, int pde, int flux_kind, int64_t K, const int32_t *deg_host, const double *geo_host,
                     const double *J_host, const double *nJ_host, const int32_t *nbr_host,
                     const int32_t *nbr_face_host, const int32_t *perm_host, const int32_t *bc_host,
