@@ -174,7 +174,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 8 :
     __shared__ __align__(16) T as[kXATiles * 64];
     __shared__ uint32_t pfw[PW];
     const int tid = threadIdx.x, w = tid >> 6, L = tid & 63;
-    for (int64_t c = xcd_block(blockIdx.x, gridDim.x); c < nchunk; c += gridDim.x) {
+    // Chunks in workgroup order, i.e. dealt round-robin to the 8 XCDs: the chunks are sorted
+    // by degree pair and a chunk's cost grows with the degrees, so giving each XCD a contiguous
+    // range (as the stage kernels do) would leave the last XCD 1.4x the mean bytes and 1.7x the
+    // mean matrix work.  Only the A tiles are reused (every pair's fit in each XCD's L2).
+    for (int64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
         const XChunk cd = chunk[c];
         const int nc = cd.code & 255, N = (cd.code >> 8) & 15, Nn = (cd.code >> 12) & 15;
         const int nf = nfp2(N), nfn = nfp2(Nn);
